@@ -1,0 +1,146 @@
+/*
+ * paising.h - C ABI of the MI355X population-annealing engine (libpaising.so).
+ *
+ * The reference (LevBarash/PAising) has no library API: its boundary is the process
+ * (argv in; files PA_results_run_<r>.dat in data<SSC|MSC>_L%d_R%d_EqSw%d_dB%f, stdout and exit code
+ * out; PAisingSSC.cu:381-806, PAisingMSC.cu:546-957).  The drop-in executables PAisingSSC
+ * and PAisingMSC (paising_amd/csrc/pa_cli.cpp) keep that boundary.  This header is the
+ * C ABI those executables, the batcher, the server and foreign-language bindings call; each
+ * entry point names the reference code it replaces.
+ *
+ * Conventions: plain C types, caller-owned opaque handles, integer error codes (PA_OK = 0),
+ * no exceptions across the boundary, one host thread per handle, work ordered on the
+ * handle's HIP stream.  The last error message of the calling thread is pa_last_error().
+ */
+#ifndef PAISING_H
+#define PAISING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pa_status {
+  PA_OK = 0,
+  PA_ERR_INVALID = 1,     /* bad parameters                                         */
+  PA_ERR_HIP = 2,         /* HIP runtime error (the reference exits with the code)  */
+  PA_ERR_NBMAX = 3,       /* more than nBmax-2 steps (PAisingSSC.cu:634-641)        */
+  PA_ERR_CAPACITY = 4,    /* population outgrew its buffers (handled internally)    */
+  PA_ERR_MEMORY = 5,      /* request does not fit the HBM budget                    */
+  PA_ERR_STATE = 6        /* call not valid in the handle's current state           */
+};
+
+enum pa_coding { PA_SSC = 0, PA_MSC = 1 };
+
+/* uniform-double convention of the RNG stream (one 32-bit word per double):
+ * PA_UNIFORM_CUDA101 = (v+1)/2^32 reproduces the CUDA >= 10.1 reference outputs,
+ * PA_UNIFORM_CUDA100 = (v+1/2)/2^32 the CUDA <= 10.0 ones. */
+enum pa_uniform { PA_UNIFORM_CUDA101 = 0, PA_UNIFORM_CUDA100 = 1 };
+
+/* Run parameters: the reference's command-line options (PAisingSSC.cu:386-400) plus the
+ * compile-time L and MSbits (PAisingSSC.cu:27, PAisingMSC.cu:34), made runtime. */
+typedef struct pa_params {
+  int32_t L;                 /* linear lattice size, even, >= 4 (reference: 64)          */
+  int32_t coding;            /* PA_SSC or PA_MSC                                          */
+  int32_t msbits;            /* MSC replicas per word: 32 or 64 (reference: 32)           */
+  int32_t Rinit;             /* -R initial population (20000)                             */
+  uint32_t eqsweeps;         /* -t sweeps per temperature (100)                           */
+  double dB;                 /* -d inverse-temperature step (0.005)                       */
+  double Binit;              /* initial beta (0)                                          */
+  double Bfin;               /* -f final beta (1)                                         */
+  uint64_t seed;             /* -s seed, as the reference passes it to curand_init         */
+  uint64_t initial_sequence; /* Philox subsequence counter at the start of the run (0)    */
+  int32_t uniform;           /* pa_uniform                                                */
+  int32_t max_steps;         /* 0 = whole schedule; otherwise stop after this many steps  */
+  int32_t trace;             /* 1 = record per-step population hashes (parity checks)     */
+  int32_t slice_bits;        /* MSC sweep slice width 8/16/32/64, 0 = autotuned           */
+  int32_t threads;           /* sweep workgroup size 128/256, 0 = autotuned               */
+  int32_t reserved;
+} pa_params;
+
+/* One row of PA_results_run_<r>.dat (PAisingSSC.cu:679-690) and the step internals. */
+typedef struct pa_row {
+  double B, E, C, M, M2, M4, BF, S; /* as printed: E/N, C/N, M/N, M2/N^2, M4/N^4, BF/N, S/N */
+  int32_t R;
+  int32_t Rprev;
+  double lnQ;
+  double q;
+  double sums[5];            /* sum E, E^2, |M|, M^2, M^4 over the population        */
+  int64_t sumE, sumM;
+  uint64_t roff_hash, spin_hash, IS_after;
+} pa_row;
+
+typedef struct pa_run pa_run;
+typedef struct pa_batch pa_batch;
+typedef struct pa_server pa_server;
+
+const char* pa_last_error(void);
+int pa_version(void);
+int pa_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes);
+
+/* Number of beta steps of the reference schedule B_i = B_{i-1} + dB while B_i <= Bfin
+ * (PAisingSSC.cu:508-510,661). */
+int pa_schedule_steps(double Binit, double dB, double Bfin);
+
+/* ---- single run: replaces main()'s run-loop body, PAisingSSC.cu:481-692 ---- */
+int pa_run_create(const pa_params* p, pa_run** out);
+/* Enqueue up to nsteps beta steps (negative = all remaining); asynchronous. */
+int pa_run_step(pa_run* r, int nsteps);
+int pa_run_sync(pa_run* r);
+int pa_run_steps_done(pa_run* r, int* done, int* total);
+/* Rows 0..n of the results file (host thermodynamics, PAisingSSC.cu:620-630,679-690). */
+int pa_run_results(pa_run* r, pa_row* rows, int max_rows, int* nrows);
+/* Philox subsequence counter after the last completed step (continues across -M runs,
+ * PAisingSSC.cu:418). */
+int pa_run_final_sequence(pa_run* r, uint64_t* is);
+/* Copy the current population to host memory: SSC R x N int8 (gA then gB per replica),
+ * MSC ceil(R/w) x N words; returns bytes needed in *bytes when dst is NULL. */
+int pa_run_population(pa_run* r, void* dst, size_t* bytes);
+/* Per-member arrays of the current population: 0 = IE, 1 = M (int32), 2 = Roff, 3 = prefix
+ * (uint32, of the last resampling), 4 = MSC active masks (uint64 per group). */
+int pa_run_array(pa_run* r, int which, void* dst, size_t* bytes);
+/* Elapsed device time of the sweep kernels and of the whole run so far (ms). */
+int pa_run_timing(pa_run* r, double* sweep_ms, double* total_ms);
+/* Time each step's sweep launches with HIP events on the run's stream (1 = on). */
+int pa_run_set_timing(pa_run* r, int on);
+/* Replay steps from a captured hipGraph (1 = on). */
+int pa_run_set_graph(pa_run* r, int on);
+int pa_run_destroy(pa_run* r);
+
+/* Whole run in one call: create, all steps, results (a reference run, PAisingSSC.cu:481-692). */
+int pa_run_all(const pa_params* p, pa_row* rows, int max_rows, int* nrows, uint64_t* final_is);
+
+/* ---- batched runs (no reference counterpart: many independent runs per launch) ---- */
+int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out);
+int pa_batch_run(pa_batch* b);            /* all runs to completion */
+int pa_batch_results(pa_batch* b, int run, pa_row* rows, int max_rows, int* nrows);
+/* Device time of the last pa_batch_run (ms). */
+int pa_batch_time(pa_batch* b, double* total_ms);
+int pa_batch_destroy(pa_batch* b);
+
+/* ---- serving (no reference counterpart): continuous batching under an HBM budget ---- */
+int pa_server_create(size_t hbm_budget_bytes, int max_runs, pa_server** out);
+int pa_server_submit(pa_server* s, const pa_params* p, int64_t* ticket);
+/* Advance the device by one batched step of every resident run, admitting queued requests
+ * as capacity frees up; returns the number of requests completed so far in *completed. */
+int pa_server_tick(pa_server* s, int* completed);
+/* 1 when the ticket is finished, 0 when queued or running. */
+int pa_server_poll(pa_server* s, int64_t ticket, int* done);
+int pa_server_results(pa_server* s, int64_t ticket, pa_row* rows, int max_rows, int* nrows);
+int pa_server_stats(pa_server* s, int* queued, int* resident, size_t* hbm_used);
+/* Steps advanced per tick and whether ticks replay a captured hipGraph. */
+int pa_server_set_chunk(pa_server* s, int steps_per_tick, int use_graph);
+/* Seconds from submission (and from admission) to completion of a finished request. */
+int pa_server_latency(pa_server* s, int64_t ticket, double* submit_to_done_s,
+                      double* admit_to_done_s);
+int pa_server_destroy(pa_server* s);
+
+/* Footprint in bytes of one run's device state (admission control). */
+size_t pa_run_footprint(const pa_params* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,325 @@
+"""MI355X population-annealing engine: Python binding of libpaising.so (C ABI, paising.h).
+
+The engine is native (gfx950 HIP kernels + C++ host engine); this module only loads the
+in-tree shared library with ctypes and converts arguments.  There is no Python or CPU
+fallback: if the library is missing or no GPU is present, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpaising.so")
+BIN_DIR = os.path.join(HERE, "bin")
+
+SSC, MSC = 0, 1
+UNIFORM_CUDA101, UNIFORM_CUDA100 = 0, 1
+OK, ERR_INVALID, ERR_HIP, ERR_NBMAX, ERR_CAPACITY, ERR_MEMORY, ERR_STATE = range(7)
+
+
+class PAError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"paising error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("L", ctypes.c_int32), ("coding", ctypes.c_int32), ("msbits", ctypes.c_int32),
+        ("Rinit", ctypes.c_int32), ("eqsweeps", ctypes.c_uint32), ("dB", ctypes.c_double),
+        ("Binit", ctypes.c_double), ("Bfin", ctypes.c_double), ("seed", ctypes.c_uint64),
+        ("initial_sequence", ctypes.c_uint64), ("uniform", ctypes.c_int32),
+        ("max_steps", ctypes.c_int32), ("trace", ctypes.c_int32), ("slice_bits", ctypes.c_int32),
+        ("threads", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+class Row(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_double), ("E", ctypes.c_double), ("C", ctypes.c_double),
+        ("M", ctypes.c_double), ("M2", ctypes.c_double), ("M4", ctypes.c_double),
+        ("BF", ctypes.c_double), ("S", ctypes.c_double), ("R", ctypes.c_int32),
+        ("Rprev", ctypes.c_int32), ("lnQ", ctypes.c_double), ("q", ctypes.c_double),
+        ("sums", ctypes.c_double * 5), ("sumE", ctypes.c_int64), ("sumM", ctypes.c_int64),
+        ("roff_hash", ctypes.c_uint64), ("spin_hash", ctypes.c_uint64),
+        ("IS_after", ctypes.c_uint64),
+    ]
+
+
+# exported entry points and their ctypes signatures (paising.h)
+_P, _I, _D, _U64, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_size_t
+_PP = ctypes.POINTER(ctypes.c_void_p)
+SIGNATURES = {
+    "pa_last_error": (ctypes.c_char_p, []),
+    "pa_version": (_I, []),
+    "pa_device_info": (_I, [ctypes.c_char_p, _SZ, _P, _P]),
+    "pa_schedule_steps": (_I, [_D, _D, _D]),
+    "pa_run_create": (_I, [_P, _PP]),
+    "pa_run_step": (_I, [_P, _I]),
+    "pa_run_sync": (_I, [_P]),
+    "pa_run_steps_done": (_I, [_P, _P, _P]),
+    "pa_run_results": (_I, [_P, _P, _I, _P]),
+    "pa_run_final_sequence": (_I, [_P, _P]),
+    "pa_run_population": (_I, [_P, _P, _P]),
+    "pa_run_array": (_I, [_P, _I, _P, _P]),
+    "pa_run_timing": (_I, [_P, _P, _P]),
+    "pa_run_set_timing": (_I, [_P, _I]),
+    "pa_run_set_graph": (_I, [_P, _I]),
+    "pa_run_destroy": (_I, [_P]),
+    "pa_run_all": (_I, [_P, _P, _I, _P, _P]),
+    "pa_batch_create": (_I, [_P, _I, _I, _PP]),
+    "pa_batch_run": (_I, [_P]),
+    "pa_batch_results": (_I, [_P, _I, _P, _I, _P]),
+    "pa_batch_time": (_I, [_P, _P]),
+    "pa_batch_destroy": (_I, [_P]),
+    "pa_server_create": (_I, [_SZ, _I, _PP]),
+    "pa_server_submit": (_I, [_P, _P, _P]),
+    "pa_server_tick": (_I, [_P, _P]),
+    "pa_server_poll": (_I, [_P, ctypes.c_int64, _P]),
+    "pa_server_results": (_I, [_P, ctypes.c_int64, _P, _I, _P]),
+    "pa_server_stats": (_I, [_P, _P, _P, _P]),
+    "pa_server_set_chunk": (_I, [_P, _I, _I]),
+    "pa_server_latency": (_I, [_P, ctypes.c_int64, _P, _P]),
+    "pa_server_destroy": (_I, [_P]),
+    "pa_run_footprint": (_SZ, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libpaising.so from this package directory (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with paising_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise PAError(rc, lib().pa_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_params(L=64, coding=MSC, msbits=32, Rinit=20000, eqsweeps=100, dB=0.005, Binit=0.0,
+                Bfin=1.0, seed=100, initial_sequence=0, uniform=UNIFORM_CUDA101, max_steps=0,
+                trace=0, slice_bits=0, threads=0):
+    return Params(L, coding, msbits, Rinit, eqsweeps, dB, Binit, Bfin, seed & (2**64 - 1),
+                  initial_sequence, uniform, max_steps, trace, slice_bits, threads, 0)
+
+
+def schedule_steps(Binit, dB, Bfin):
+    return lib().pa_schedule_steps(Binit, dB, Bfin)
+
+
+def footprint(params):
+    return lib().pa_run_footprint(ctypes.byref(params))
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int(0)
+    hbm = ctypes.c_size_t(0)
+    _check(lib().pa_device_info(name, 256, ctypes.byref(cu), ctypes.byref(hbm)))
+    return name.value.decode(), cu.value, hbm.value
+
+
+def _rows(buf, n):
+    return [buf[i] for i in range(n)]
+
+
+def run_all(params, max_rows=10000):
+    """One complete run; returns (rows, final Philox subsequence counter)."""
+    buf = (Row * max_rows)()
+    n = ctypes.c_int(0)
+    fin = ctypes.c_uint64(0)
+    _check(lib().pa_run_all(ctypes.byref(params), buf, max_rows, ctypes.byref(n), ctypes.byref(fin)))
+    return _rows(buf, n.value), fin.value
+
+
+class Run:
+    """A run kept on the device between calls (stepwise access for tests and timing)."""
+
+    def __init__(self, params):
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(lib().pa_run_create(ctypes.byref(params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pa_run_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=-1):
+        _check(lib().pa_run_step(self.h, n))
+
+    def sync(self):
+        _check(lib().pa_run_sync(self.h))
+
+    def set_timing(self, on=True):
+        _check(lib().pa_run_set_timing(self.h, int(on)))
+
+    def set_graph(self, on=True):
+        _check(lib().pa_run_set_graph(self.h, int(on)))
+
+    def timing(self):
+        s = ctypes.c_double(0)
+        t = ctypes.c_double(0)
+        _check(lib().pa_run_timing(self.h, ctypes.byref(s), ctypes.byref(t)))
+        return s.value, t.value
+
+    def results(self, max_rows=10000):
+        buf = (Row * max_rows)()
+        n = ctypes.c_int(0)
+        _check(lib().pa_run_results(self.h, buf, max_rows, ctypes.byref(n)))
+        return _rows(buf, n.value)
+
+    def final_sequence(self):
+        v = ctypes.c_uint64(0)
+        _check(lib().pa_run_final_sequence(self.h, ctypes.byref(v)))
+        return v.value
+
+    def population(self):
+        """SSC: int8 array (R, N); MSC: uint32/uint64 array (groups, N)."""
+        nb = ctypes.c_size_t(0)
+        _check(lib().pa_run_population(self.h, None, ctypes.byref(nb)))
+        N = self.params.L * self.params.L
+        if self.params.coding == SSC:
+            a = np.empty(nb.value, dtype=np.int8)
+        else:
+            a = np.empty(nb.value // (self.params.msbits // 8),
+                         dtype=np.uint32 if self.params.msbits == 32 else np.uint64)
+        _check(lib().pa_run_population(self.h, _ptr(a), ctypes.byref(nb)))
+        return a.reshape(-1, N)
+
+    def array(self, which):
+        """0 IE, 1 M (int32); 2 Roff, 3 prefix (uint32); 4 MSC active masks (uint64)."""
+        nb = ctypes.c_size_t(0)
+        _check(lib().pa_run_array(self.h, which, None, ctypes.byref(nb)))
+        dt = {0: np.int32, 1: np.int32, 2: np.uint32, 3: np.uint32, 4: np.uint64}[which]
+        a = np.empty(nb.value // np.dtype(dt).itemsize, dtype=dt)
+        _check(lib().pa_run_array(self.h, which, _ptr(a), ctypes.byref(nb)))
+        return a
+
+
+class Batch:
+    """Independent runs advanced together by the same launches (optionally a hipGraph)."""
+
+    def __init__(self, params_list, use_graph=True):
+        arr = (Params * len(params_list))(*params_list)
+        h = ctypes.c_void_p()
+        _check(lib().pa_batch_create(arr, len(params_list), int(use_graph), ctypes.byref(h)))
+        self.h = h
+        self.n = len(params_list)
+
+    def run(self):
+        _check(lib().pa_batch_run(self.h))
+
+    def time_ms(self):
+        t = ctypes.c_double(0)
+        _check(lib().pa_batch_time(self.h, ctypes.byref(t)))
+        return t.value
+
+    def results(self, i, max_rows=10000):
+        buf = (Row * max_rows)()
+        n = ctypes.c_int(0)
+        _check(lib().pa_batch_results(self.h, i, buf, max_rows, ctypes.byref(n)))
+        return _rows(buf, n.value)
+
+    def close(self):
+        if self.h:
+            lib().pa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server: submit runs, tick, poll, fetch results."""
+
+    def __init__(self, hbm_budget_bytes, max_runs=256, steps_per_tick=8, use_graph=True):
+        h = ctypes.c_void_p()
+        _check(lib().pa_server_create(hbm_budget_bytes, max_runs, ctypes.byref(h)))
+        self.h = h
+        _check(lib().pa_server_set_chunk(self.h, steps_per_tick, int(use_graph)))
+
+    def submit(self, params):
+        t = ctypes.c_int64(0)
+        _check(lib().pa_server_submit(self.h, ctypes.byref(params), ctypes.byref(t)))
+        return t.value
+
+    def tick(self):
+        c = ctypes.c_int(0)
+        _check(lib().pa_server_tick(self.h, ctypes.byref(c)))
+        return c.value
+
+    def done(self, ticket):
+        d = ctypes.c_int(0)
+        _check(lib().pa_server_poll(self.h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def results(self, ticket, max_rows=10000):
+        buf = (Row * max_rows)()
+        n = ctypes.c_int(0)
+        _check(lib().pa_server_results(self.h, ticket, buf, max_rows, ctypes.byref(n)))
+        return _rows(buf, n.value)
+
+    def latency(self, ticket):
+        a = ctypes.c_double(0)
+        b = ctypes.c_double(0)
+        _check(lib().pa_server_latency(self.h, ticket, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def stats(self):
+        q = ctypes.c_int(0)
+        r = ctypes.c_int(0)
+        u = ctypes.c_size_t(0)
+        _check(lib().pa_server_stats(self.h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(u)))
+        return q.value, r.value, u.value
+
+    def close(self):
+        if self.h:
+            lib().pa_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def row_fields(row):
+    return [row.B, row.E, row.C, row.M, row.M2, row.M4, row.BF, row.S, row.R, row.lnQ]
+
+
+def format_rows(rows, precision=11):
+    """Results-file text as the reference writes it (ostream precision P, SSC:679-690)."""
+    out = []
+    for r in rows:
+        f = row_fields(r)
+        out.append(" ".join(["%.*g" % (precision, v) for v in f[:8]] + ["%d" % f[8],
+                                                                        "%.*g" % (precision, f[9])]) + "\n")
+    return "".join(out)

@@ -1,0 +1,197 @@
+// Drop-in executables PAisingSSC (PA_CLI_CODING=0) and PAisingMSC (PA_CLI_CODING=1).
+//
+// Same options, defaults, directory naming, results-file format, stdout lines and exit
+// codes as the reference main() (PAisingSSC.cu:338-806, PAisingMSC.cu:503-957), with the
+// compile-time constants made runtime options:
+//   -L size   (PAisingSSC.cu:27, default 64)
+//   -w bits   (PAisingMSC.cu:34 MSbits, 32 or 64, default 32; MSC only)
+//   -u conv   uniform-double convention: 0 = CUDA >= 10.1 (default), 1 = CUDA <= 10.0
+// The GPU work goes through the C ABI of paising.h (pa_run_all per run).
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <vector>
+
+#include "paising.h"
+
+#ifndef PA_CLI_CODING
+#define PA_CLI_CODING 0
+#endif
+
+namespace {
+
+// BSD-style getopt as used by the reference (PAisingSSC.cu:338-365): prints
+// "illegal option -- c" / "option requires an argument -- c" and returns '?'.
+char* opt_arg;
+int opt_ind = 1, opt_opt;
+
+int get_opt(int nargc, char* const nargv[], const char* ostr) {
+  static const char* place = "";
+  const char* oli;
+  if (!*place) {
+    if (opt_ind >= nargc || *(place = nargv[opt_ind]) != '-') { place = ""; return -1; }
+    if (place[1] && *++place == '-') { ++opt_ind; place = ""; return -1; }
+  }
+  if ((opt_opt = (int)*place++) == (int)':' || !(oli = strchr(ostr, opt_opt))) {
+    if (opt_opt == (int)'-') return -1;
+    if (!*place) ++opt_ind;
+    printf("illegal option -- %c\n", opt_opt);
+    return (int)'?';
+  }
+  if (*++oli != ':') {
+    opt_arg = nullptr;
+    if (!*place) ++opt_ind;
+  } else {
+    if (*place) {
+      opt_arg = (char*)place;
+    } else if (nargc <= ++opt_ind) {
+      place = "";
+      printf("option requires an argument -- %c\n", opt_opt);
+      return (int)'?';
+    } else {
+      opt_arg = nargv[opt_ind];
+    }
+    place = "";
+    ++opt_ind;
+  }
+  return opt_opt;
+}
+
+void usage(const char* prog) {
+  std::cout << " Usage: " << prog << " [options]\n"
+            << " Note: all of the options are optional. Default parameter values are listed in the head of the source code. \n"
+            << " Possible command line options are:\n\n"
+            << " -R Rinit           ( Rinit           = initial size of population of replicas )\n"
+            << " -t EQsweeps        ( EQsweeps        = number of equilibration sweeps )\n"
+            << " -d dBinit          ( dBinit          = inverse temperature step )\n"
+            << " -f Bfin            ( Bfin            = final value of inverse temperature )\n"
+            << " -M runs            ( runs            = number of population annealing algorithm independent runs )\n"
+            << " -s RNGseed         ( RNGseed         = seed for random number generation )\n"
+            << " -P OutputPrecision ( OutputPrecision = precision (number of digits) of the output )\n"
+            << " -o dataDirectory   ( dataDirectory   = data directory name )\n"
+            << " -L size            ( size            = linear lattice size, even )\n"
+#if PA_CLI_CODING == 1
+            << " -w bits            ( bits            = replicas per multi-spin word, 32 or 64 )\n"
+#endif
+            << " -u convention      ( convention      = uniform-double mapping: 0 CUDA>=10.1, 1 CUDA<=10.0 )\n";
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const bool msc = PA_CLI_CODING == 1;
+  const char* prog = msc ? "PAisingMSC" : "PAisingSSC";
+  char dataDir[512];
+  int optdir = 0;
+  long long seed_opt = (long long)time(NULL);
+  unsigned int EQsweeps = 100;
+  double Bfin = 1.0, dBinit = 0.005;
+  int Rinit = 20000, runs = 1, OutputPrecision = 11, L = 64, bits = 32, uconv = 0;
+  int optc, opti;
+  double optf;
+  while ((optc = get_opt(argc, argv, msc ? "R:t:d:f:M:s:P:o:L:w:u:?" : "R:t:d:f:M:s:P:o:L:u:?")) != -1) {
+    switch (optc) {
+      case 'R': opti = atoi(opt_arg); if (opti) Rinit = opti; break;
+      case 't': opti = atoi(opt_arg); EQsweeps = opti; break;
+      case 'd': optf = atof(opt_arg); if (optf > 0.0) dBinit = optf; break;
+      case 'f': optf = atof(opt_arg); if (optf > 0.0) Bfin = optf; break;
+      case 'M': opti = atoi(opt_arg); if (opti) runs = opti; break;
+      case 's': opti = atoi(opt_arg); if (opti) seed_opt = opti; break;
+      case 'P': opti = atoi(opt_arg); if (opti) OutputPrecision = opti; break;
+      case 'o':
+        if (opt_arg[strlen(opt_arg) - 1] == '/') snprintf(dataDir, sizeof dataDir, "%s", opt_arg);
+        else snprintf(dataDir, sizeof dataDir, "%s/", opt_arg);
+        optdir = 1;
+        break;
+      case 'L': opti = atoi(opt_arg); if (opti) L = opti; break;
+      case 'w': opti = atoi(opt_arg); if (opti) bits = opti; break;
+      case 'u': uconv = atoi(opt_arg); break;
+      case '?': usage(prog); return 1;
+    }
+  }
+  if (opt_ind < argc) {
+    for (opti = opt_ind; opti < argc; opti++) fprintf(stderr, "Non-option argument %s\n", argv[opti]);
+    return 1;
+  }
+  // SSC keeps the seed in an int (PAisingSSC.cu:384), MSC in an unsigned long long
+  // (PAisingMSC.cu:549); both reach curand_init as unsigned long long.
+  const int seed_int = (int)seed_opt;
+  const unsigned long long seed = msc ? (unsigned long long)seed_opt : (unsigned long long)(long long)seed_int;
+  if (!optdir)
+    snprintf(dataDir, sizeof dataDir, "./data%s_L%d_R%d_EqSw%d_dB%f/", msc ? "MSC" : "SSC", L, Rinit,
+             EQsweeps, dBinit);
+  mkdir(dataDir, 0777);
+
+  pa_params p;
+  memset(&p, 0, sizeof p);
+  p.L = L;
+  p.coding = msc ? PA_MSC : PA_SSC;
+  p.msbits = bits;
+  p.Rinit = Rinit;
+  p.eqsweeps = EQsweeps;
+  p.dB = dBinit;
+  p.Binit = 0.0;
+  p.Bfin = Bfin;
+  p.seed = seed;
+  p.uniform = uconv ? PA_UNIFORM_CUDA100 : PA_UNIFORM_CUDA101;
+
+  if (msc) std::cout << "RNG initial seed: " << seed << "\n";
+  else std::cout << "RNG initial seed: " << seed_int << "\n";
+  const size_t fp = pa_run_footprint(&p);
+  const int N = L * L;
+  if (fp == 0) {
+    fprintf(stderr, "%s\n", pa_last_error());
+    return 1;
+  }
+  std::cout << "Memory use of one replica: " << (double)fp / Rinit / 1024.0 << " kB \n";
+  std::cout << "Memory use of the entire population of " << Rinit
+            << " replicas: " << (double)fp / 1024.0 / 1024.0 << " MB \n";
+  fflush(stdout);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  double totPop = 0.0;
+  unsigned long long IS = 0;
+  std::vector<pa_row> rows(10000);
+  for (int r = 0; r < runs; ++r) {
+    p.initial_sequence = IS;
+    int nrows = 0;
+    uint64_t is_end = 0;
+    const int rc = pa_run_all(&p, rows.data(), (int)rows.size(), &nrows, &is_end);
+    if (rc == PA_ERR_NBMAX) {
+      fprintf(stderr,
+              "Error: number of temperature steps exceeds nBmax=%d.\n Please consider increasing the "
+              "inverse temperature step or increasing the value of nBmax.\n",
+              10000);
+      return 1;
+    }
+    if (rc) {
+      fprintf(stderr, "%s\n", pa_last_error());
+      return rc;
+    }
+    IS = is_end;
+    for (int i = 1; i < nrows; ++i) totPop += rows[i].R;
+    char str[700];
+    snprintf(str, sizeof str, "%sPA_results_run_%d.dat", dataDir, r);
+    std::ofstream results(str);
+    results.precision(OutputPrecision);
+    for (int i = 0; i < nrows; ++i) {
+      const pa_row& w = rows[i];
+      results << w.B << " " << w.E << " " << w.C << " " << w.M << " " << w.M2 << " " << w.M4 << " "
+              << w.BF << " " << w.S << " " << w.R << " " << w.lnQ << "\n";
+    }
+  }
+  const double Etime =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  std::cout << "Elapsed time: " << std::setprecision(8) << Etime / 1000 << " s\n";
+  std::cout << "Time per spin-flip: " << std::setprecision(8) << Etime * 1e6 / EQsweeps / N / totPop
+            << " ns\n";
+  return 0;
+}

@@ -1,0 +1,962 @@
+// Host engine: schedules, exact acceptance thresholds, device buffers, batched step launches
+// (optionally replayed from a hipGraph), host thermodynamics and the C ABI of paising.h.
+//
+// Reference counterparts (PAisingSSC.cu = SSC, PAisingMSC.cu = MSC):
+//   schedule B_i = B_{i-1} + dB while B_i <= Bfin      SSC:508-510,661 / MSC:670-672,812
+//   Boltzmann tables                                   SSC:512-516     / MSC:674-675
+//   thermodynamics E, C, M, M2, M4, BF, S              SSC:620-630     / MSC:772-782
+//   results row                                        SSC:679-690     / MSC:829-840
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/paising.h"
+#include "pa_dev.h"
+#include "pa_prims.h"
+
+extern "C" hipError_t pa_launch_init(int coding, int w, int sw, int nt, int nblocks, size_t lds,
+                                     hipStream_t s, const pa::RunDesc* runs, const int32_t* offs,
+                                     int nruns, const uint64_t* is0);
+extern "C" hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nblocks,
+                                     size_t lds, hipStream_t s, const pa::RunDesc* runs,
+                                     const int32_t* offs, int nruns);
+extern "C" hipError_t pa_prepare_kernels(void);
+
+namespace {
+
+using namespace pa;
+
+constexpr int kNBmax = 10000;            // nBmax, SSC:57
+constexpr size_t kLdsLimit = 163840;     // gfx950 LDS per workgroup
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHECK(expr)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(PA_ERR_HIP, std::string("GPUassert: ") + hipGetErrorString(e_) + " ; " \
+                                  + __FILE__ + " ; line " + std::to_string(__LINE__));   \
+  } while (0)
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Reference schedule: FP64 accumulation of dB from Binit while <= Bfin.
+std::vector<double> schedule(double Binit, double dB, double Bfin, int max_steps) {
+  std::vector<double> B{Binit};
+  double b = Binit + dB;
+  while (b <= Bfin) {
+    if (max_steps > 0 && (int)B.size() - 1 >= max_steps) break;
+    B.push_back(b);
+    if ((int)B.size() > kNBmax + 1) break;
+    b = B.back() + dB;
+  }
+  return B;
+}
+
+// V(T) = #{v in [0, 2^32) : uniform_double(v) < T}; uniform_double is non-decreasing in v.
+uint64_t ssc_threshold(double T, int umap) {
+  uint64_t lo = 0, hi = 1ull << 32;  // find the smallest v with u(v) >= T (2^32 if none)
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (uniform_double((uint32_t)mid, umap) >= T) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+struct SweepVariant {
+  int coding, w, sw, nt;
+  bool operator<(const SweepVariant& o) const {
+    return std::tie(coding, w, sw, nt) < std::tie(o.coding, o.w, o.sw, o.nt);
+  }
+};
+
+struct RunHost {
+  pa_params p{};
+  int w = 1, N = 0, n_steps = 0, cap_pop = 0, cap_units = 0;
+  int R0 = 0;
+  int sw = 0, nt = 0;
+  std::vector<double> B;
+  size_t footprint = 0;
+  // device memory (one allocation carved into pieces)
+  void* arena = nullptr;
+  RunDesc desc{};
+  std::vector<StepOut> out_h;
+  RunState st_h{};
+  bool results_valid = false;
+
+  ~RunHost() {
+    if (arena) (void)hipFree(arena);
+  }
+  int blocks(int kind) const {
+    switch (kind) {
+      case kKUnits: return cap_units;
+      case kKChunks: return ceil_div(cap_pop, kRedBlock);
+      case kKGroupChunks: return ceil_div(cap_units, kRedBlock);
+      case kKAvg: return p.coding == kSSC ? ceil_div(cap_pop, kRedBlock)
+                                          : ceil_div(cap_units, kRedBlock) * w;
+    }
+    return 0;
+  }
+};
+
+// Validate, size and compute the per-step tables of a run (no device work).
+int plan_run(const pa_params& p, RunHost& h, double cap_sigma) {
+  if (p.L < 4 || (p.L & 1) || p.L > 1024) return fail(PA_ERR_INVALID, "L must be even, 4..1024");
+  if (p.coding != PA_SSC && p.coding != PA_MSC) return fail(PA_ERR_INVALID, "coding");
+  if (p.coding == PA_MSC && p.msbits != 32 && p.msbits != 64)
+    return fail(PA_ERR_INVALID, "msbits must be 32 or 64");
+  if (p.Rinit < 1) return fail(PA_ERR_INVALID, "Rinit < 1");
+  if (!(p.dB > 0.0) || p.Binit < 0.0) return fail(PA_ERR_INVALID, "dB must be > 0, Binit >= 0");
+  if (p.uniform != PA_UNIFORM_CUDA101 && p.uniform != PA_UNIFORM_CUDA100)
+    return fail(PA_ERR_INVALID, "uniform");
+  h.p = p;
+  h.w = p.coding == PA_SSC ? 1 : p.msbits;
+  h.N = p.L * p.L;
+  h.B = schedule(p.Binit, p.dB, p.Bfin, p.max_steps);
+  h.n_steps = (int)h.B.size() - 1;
+  if (h.n_steps + 1 >= kNBmax)
+    return fail(PA_ERR_NBMAX, "Error: number of temperature steps exceeds nBmax=10000.");
+  h.R0 = p.Rinit;
+  const double margin = cap_sigma * std::sqrt((double)p.Rinit) + 64.0;
+  h.cap_pop = p.Rinit + (int)margin;
+  if (p.coding == PA_SSC) {
+    h.cap_units = h.cap_pop;
+  } else {
+    h.cap_units = ceil_div(h.cap_pop, h.w);
+    h.cap_pop = h.cap_units * h.w;
+  }
+  // sweep variant
+  const int Nh = h.N / 2;
+  const bool aligned = (Nh % 512) == 0;
+  h.nt = p.threads ? p.threads : (aligned ? 256 : 128);
+  if (h.nt != 128 && h.nt != 256) return fail(PA_ERR_INVALID, "threads must be 128 or 256");
+  if (!aligned) h.nt = 128;
+  if (p.coding == PA_MSC) {
+    int sw = p.slice_bits;
+    if (sw == 0) {
+      sw = h.w;
+      while (sw > 8 && (size_t)h.N * (sw / 8) > 96 * 1024) sw /= 2;
+    }
+    if (sw != 8 && sw != 16 && sw != 32 && sw != 64) return fail(PA_ERR_INVALID, "slice_bits");
+    if (sw > h.w) sw = h.w;
+    if ((size_t)h.N * (sw / 8) + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice slice exceeds LDS");
+    h.sw = sw;
+  } else {
+    if ((size_t)h.N + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice exceeds LDS");
+    h.sw = 8;
+  }
+  return PA_OK;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Byte layout of a run's arena; returns total size.  Pointers are filled when base != null.
+size_t layout(RunHost& h, uint8_t* base) {
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes); return base ? base + at : nullptr; };
+  const size_t unit_bytes = h.p.coding == PA_SSC ? (size_t)h.N : (size_t)h.N * (h.w / 8);
+  RunDesc& d = h.desc;
+  d.pop[0] = take((size_t)h.cap_units * unit_bytes);
+  d.pop[1] = take((size_t)h.cap_units * unit_bytes);
+  d.amask[0] = (uint64_t*)take((size_t)h.cap_units * 8);
+  d.amask[1] = (uint64_t*)take((size_t)h.cap_units * 8);
+  d.IE = (int32_t*)take((size_t)h.cap_pop * 4);
+  d.M = (int32_t*)take((size_t)h.cap_pop * 4);
+  d.Roff = (uint32_t*)take((size_t)h.cap_pop * 4);
+  d.pre = (uint32_t*)take((size_t)h.cap_pop * 4);
+  d.cblk = (uint32_t*)take((size_t)h.blocks(kKChunks) * 4);
+  d.qblk = (double*)take((size_t)h.blocks(kKGroupChunks) * 8 + 8 * (size_t)h.blocks(kKChunks));
+  d.avblk = (double*)take((size_t)h.blocks(kKAvg) * 5 * 8);
+  d.iblk = (int64_t*)take((size_t)h.blocks(kKAvg) * 2 * 8);
+  d.hblk = (uint64_t*)take(64);
+  d.beta = (const double*)take((size_t)(h.n_steps + 1) * 8);
+  d.dbeta = (const double*)take((size_t)(h.n_steps + 1) * 8);
+  d.thr = (const uint64_t*)take((size_t)(h.n_steps + 1) * 3 * 8);
+  d.out = (StepOut*)take((size_t)(h.n_steps + 1) * sizeof(StepOut));
+  d.st = (RunState*)take(sizeof(RunState));
+  return o;
+}
+
+// Allocate and upload a planned run's state (tables, initial RunState); no kernels.
+int materialize(RunHost& h) {
+  h.footprint = layout(h, nullptr);
+  HIPCHECK(hipMalloc(&h.arena, h.footprint));
+  layout(h, (uint8_t*)h.arena);
+  RunDesc& d = h.desc;
+  d.L = h.p.L; d.Ld2 = h.p.L / 2; d.N = h.N; d.Nh = h.N / 2;
+  d.coding = h.p.coding; d.w = h.w; d.Rinit = h.p.Rinit; d.n_steps = h.n_steps;
+  d.cap_units = h.cap_units; d.umap = h.p.uniform; d.eqsweeps = h.p.eqsweeps;
+  d.trace = h.p.trace; d.q128 = 128 / d.Ld2; d.r128 = 128 % d.Ld2; d.seed = h.p.seed;
+  std::vector<double> dbeta(h.n_steps + 1, 0.0);
+  std::vector<uint64_t> thr((size_t)(h.n_steps + 1) * 3, 0);
+  for (int i = 1; i <= h.n_steps; ++i) {
+    const double b = h.B[i];
+    dbeta[i] = h.B[i] - h.B[i - 1];
+    if (h.p.coding == PA_SSC) {
+      // T[idxT0 + 2 idxT1] = min(1, exp(-2 (2 idxT0 - 1)(2 idxT1 - 4) B)), SSC:512-516:
+      // dE <= 0 -> 1, dE = 4 -> exp(-4B), dE = 8 -> exp(-8B)
+      thr[3 * i + 0] = ssc_threshold(1.0, h.p.uniform);
+      thr[3 * i + 1] = ssc_threshold(std::min(1.0, std::exp(-4 * b)), h.p.uniform);
+      thr[3 * i + 2] = ssc_threshold(std::min(1.0, std::exp(-8 * b)), h.p.uniform);
+    } else {
+      thr[3 * i + 0] = (uint32_t)std::ceil(4294967296. * std::exp(-4 * b));  // MSC:674
+      thr[3 * i + 1] = (uint32_t)std::ceil(4294967296. * std::exp(-8 * b));  // MSC:675
+    }
+  }
+  HIPCHECK(hipMemcpy((void*)d.beta, h.B.data(), (h.n_steps + 1) * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy((void*)d.dbeta, dbeta.data(), (h.n_steps + 1) * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy((void*)d.thr, thr.data(), thr.size() * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemset(d.out, 0, (h.n_steps + 1) * sizeof(StepOut)));
+  RunState st{};
+  st.step = 1;
+  st.R = h.p.Rinit;
+  st.Rnew = h.p.Rinit;
+  st.buf = 0;
+  const int units0 = h.p.coding == PA_SSC ? h.p.Rinit : ceil_div(h.p.Rinit, h.w);
+  st.IS = h.p.initial_sequence + (uint64_t)units0 * kStreamsPerUnit;  // SSC:486 / MSC:647
+  st.Emean = 0.0;                                                     // SSC:483
+  st.lnQ = 0.0;
+  st.overflow = 0;
+  st.active = h.n_steps > 0 ? 1 : 0;
+  HIPCHECK(hipMemcpy(d.st, &st, sizeof(RunState), hipMemcpyHostToDevice));
+  h.st_h = st;
+  return PA_OK;
+}
+
+struct Group {  // runs sharing one sweep kernel variant
+  SweepVariant v;
+  std::vector<int> members;
+  RunDesc* desc_d = nullptr;
+  int32_t* offs_d = nullptr;
+  int units_blocks = 0;  // total kKUnits blocks
+  size_t lds = 0;
+};
+
+struct Batch {
+  std::vector<RunHost*> runs;  // not owned when borrowed by the server
+  std::vector<std::unique_ptr<RunHost>> owned;
+  hipStream_t stream = nullptr;
+  RunDesc* desc_d = nullptr;
+  int32_t* offs_d = nullptr;
+  uint64_t* is0_d = nullptr;
+  int blocks[kKindCount] = {0, 0, 0, 0};
+  std::vector<Group> groups;
+  bool any_trace = false;
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int steps_launched = 0;
+  int max_steps = 0;
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
+  double sweep_ms = 0.0, total_ms = 0.0;
+  std::vector<void*> table_allocs;
+
+  ~Batch() { release(); }
+  void release_tables() {
+    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    for (void* p : table_allocs) (void)hipFree(p);
+    table_allocs.clear();
+    groups.clear();
+    init_is0.clear();
+  }
+  void release() {
+    release_tables();
+    if (ev0) { (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1); ev0 = ev1 = nullptr; }
+    if (evs0) { (void)hipEventDestroy(evs0); (void)hipEventDestroy(evs1); evs0 = evs1 = nullptr; }
+    if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
+  }
+
+  int alloc_table(void** p, size_t bytes) {
+    HIPCHECK(hipMalloc(p, bytes));
+    table_allocs.push_back(*p);
+    return PA_OK;
+  }
+
+  // Build descriptor and prefix tables for the current run set.
+  int build_tables() {
+    release_tables();
+    const int n = (int)runs.size();
+    if (n == 0) return PA_OK;
+    std::vector<RunDesc> descs(n);
+    std::vector<int32_t> offs((size_t)kKindCount * (n + 1), 0);
+    std::vector<uint64_t> is0(n);
+    any_trace = false;
+    max_steps = 0;
+    for (int k = 0; k < kKindCount; ++k) {
+      int acc = 0;
+      for (int r = 0; r < n; ++r) {
+        offs[(size_t)k * (n + 1) + r] = acc;
+        acc += runs[r]->blocks(k);
+      }
+      offs[(size_t)k * (n + 1) + n] = acc;
+      blocks[k] = acc;
+    }
+    for (int r = 0; r < n; ++r) {
+      descs[r] = runs[r]->desc;
+      is0[r] = runs[r]->p.initial_sequence;
+      any_trace |= runs[r]->p.trace != 0;
+      max_steps = std::max(max_steps, runs[r]->n_steps);
+    }
+    int rc;
+    if ((rc = alloc_table((void**)&desc_d, n * sizeof(RunDesc)))) return rc;
+    if ((rc = alloc_table((void**)&offs_d, offs.size() * 4))) return rc;
+    if ((rc = alloc_table((void**)&is0_d, n * 8))) return rc;
+    HIPCHECK(hipMemcpy(desc_d, descs.data(), n * sizeof(RunDesc), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(offs_d, offs.data(), offs.size() * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(is0_d, is0.data(), n * 8, hipMemcpyHostToDevice));
+    // sweep groups (by kernel variant); the init launches use the same grouping
+    {
+      std::map<SweepVariant, std::vector<int>> gm;
+      for (int r = 0; r < n; ++r) {
+        const RunHost& h = *runs[r];
+        SweepVariant v{h.p.coding, h.w, h.sw, h.nt};
+        gm[v].push_back(r);
+      }
+      for (auto& kv : gm) {
+        Group g;
+        g.v = kv.first;
+        g.members = kv.second;
+        const int m = (int)g.members.size();
+        std::vector<RunDesc> gd(m);
+        std::vector<int32_t> go((size_t)kKindCount * (m + 1), 0);
+        std::vector<uint64_t> gis(m);
+        int acc = 0;
+        size_t lds = 0;
+        for (int i = 0; i < m; ++i) {
+          const RunHost& h = *runs[g.members[i]];
+          gd[i] = h.desc;
+          gis[i] = h.p.initial_sequence;
+          go[(size_t)kKUnits * (m + 1) + i] = acc;
+          acc += h.cap_units;
+          const size_t l = h.p.coding == PA_SSC ? (size_t)h.N : (size_t)h.N * (h.sw / 8);
+          lds = std::max(lds, l);
+        }
+        go[(size_t)kKUnits * (m + 1) + m] = acc;
+        g.units_blocks = acc;
+        g.lds = lds;
+        if ((rc = alloc_table((void**)&g.desc_d, m * sizeof(RunDesc)))) return rc;
+        if ((rc = alloc_table((void**)&g.offs_d, go.size() * 4))) return rc;
+        HIPCHECK(hipMemcpy(g.desc_d, gd.data(), m * sizeof(RunDesc), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(g.offs_d, go.data(), go.size() * 4, hipMemcpyHostToDevice));
+        uint64_t* gis_d = nullptr;
+        if ((rc = alloc_table((void**)&gis_d, m * 8))) return rc;
+        HIPCHECK(hipMemcpy(gis_d, gis.data(), m * 8, hipMemcpyHostToDevice));
+        groups.push_back(g);
+        init_is0.push_back(gis_d);
+      }
+    }
+    return PA_OK;
+  }
+  std::vector<uint64_t*> init_is0;
+
+  int launch_init() {
+    for (size_t i = 0; i < groups.size(); ++i) {
+      const Group& g = groups[i];
+      HIPCHECK(pa_launch_init(g.v.coding, g.v.w, g.v.sw, g.v.nt, g.units_blocks, g.lds, stream,
+                              g.desc_d, g.offs_d, (int)g.members.size(), init_is0[i]));
+    }
+    return PA_OK;
+  }
+
+  // One beta step of every active run (8 launches + one sweep launch per variant).
+  int launch_step(bool time_sweep) {
+    const int n = (int)runs.size();
+    HIPCHECK(pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n));
+    HIPCHECK(pa_launch_step(1, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
+    HIPCHECK(pa_launch_step(2, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
+    HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, blocks[kKUnits], 0, stream, desc_d, offs_d, n));
+    if (time_sweep) HIPCHECK(hipEventRecord(evs0, stream));
+    for (const Group& g : groups) {
+      const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;
+      HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.units_blocks * ns, g.lds, stream,
+                              g.desc_d, g.offs_d, (int)g.members.size()));
+    }
+    if (time_sweep) HIPCHECK(hipEventRecord(evs1, stream));
+    HIPCHECK(pa_launch_step(5, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n));
+    if (any_trace) HIPCHECK(pa_launch_step(6, 0, 0, 0, 0, blocks[kKUnits], 0, stream, desc_d, offs_d, n));
+    HIPCHECK(pa_launch_step(7, 0, 0, 0, 0, n, 0, stream, desc_d, offs_d, n));
+    return PA_OK;
+  }
+
+  int ensure_graph() {
+    if (gexec) return PA_OK;
+    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(false);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &gr);
+    if (rc) return rc;
+    HIPCHECK(e);
+    graph = gr;
+    HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    return PA_OK;
+  }
+
+  int run_steps(int k) {
+    for (int i = 0; i < k; ++i) {
+      if (use_graph) {
+        int rc = ensure_graph();
+        if (rc) return rc;
+        HIPCHECK(hipGraphLaunch(gexec, stream));
+      } else if (timing) {
+        int rc = launch_step(true);
+        if (rc) return rc;
+        HIPCHECK(hipEventSynchronize(evs1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, evs0, evs1));
+        sweep_ms += ms;
+      } else {
+        int rc = launch_step(false);
+        if (rc) return rc;
+      }
+      ++steps_launched;
+    }
+    return PA_OK;
+  }
+
+  int create_stream() {
+    static bool prepared = false;
+    if (!prepared) {
+      HIPCHECK(pa_prepare_kernels());
+      prepared = true;
+    }
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&ev0));
+    HIPCHECK(hipEventCreate(&ev1));
+    HIPCHECK(hipEventCreate(&evs0));
+    HIPCHECK(hipEventCreate(&evs1));
+    return PA_OK;
+  }
+};
+
+// Copy step records and state back; compute the results rows like SSC:620-630 and 679-690.
+int fetch(RunHost& h, hipStream_t s) {
+  h.out_h.resize(h.n_steps + 1);
+  HIPCHECK(hipMemcpyAsync(h.out_h.data(), h.desc.out, (h.n_steps + 1) * sizeof(StepOut),
+                          hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(&h.st_h, h.desc.st, sizeof(RunState), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  if (h.st_h.overflow)
+    return fail(PA_ERR_CAPACITY, "population outgrew its buffers");
+  h.results_valid = true;
+  return PA_OK;
+}
+
+int rows_of(const RunHost& h, pa_row* rows, int max_rows, int* nrows) {
+  const int done = h.st_h.step - 1;  // completed steps
+  const int N = h.N;
+  double sumlnQ = 0.0;
+  const int n = std::min(done + 1, max_rows);
+  for (int i = 0; i < n; ++i) {
+    pa_row& row = rows[i];
+    memset(&row, 0, sizeof(row));
+    row.B = h.B[i];
+    if (i == 0) {
+      // the reference prints never-assigned values for row 0 (they are 0 in its outputs)
+      row.BF = (-N * std::log(2.0)) / N;
+      row.R = h.p.Rinit;
+      continue;
+    }
+    const StepOut& o = h.out_h[i];
+    const double R = (double)o.R;
+    const double E = o.sums[0] / o.R;
+    const double C = (o.sums[1] / o.R - E * E) * h.B[i] * h.B[i];
+    const double M = o.sums[2] / o.R;
+    const double M2 = o.sums[3] / o.R;
+    const double M4 = o.sums[4] / o.R;
+    (void)R;
+    sumlnQ -= o.lnQ;
+    const double BF = -N * std::log(2.0) + sumlnQ;
+    const double S = h.B[i] * E - BF;
+    row.E = E / N;
+    row.C = C / N;
+    row.M = M / N;
+    row.M2 = M2 / N / N;
+    row.M4 = M4 / N / N / N / N;
+    row.BF = BF / N;
+    row.S = S / N;
+    row.R = o.R;
+    row.Rprev = o.Rprev;
+    row.lnQ = o.lnQ;
+    row.q = o.q;
+    for (int k = 0; k < 5; ++k) row.sums[k] = o.sums[k];
+    row.sumE = o.sumE;
+    row.sumM = o.sumM;
+    row.roff_hash = o.roff_hash;
+    row.spin_hash = o.spin_hash;
+    row.IS_after = o.IS_after;
+  }
+  *nrows = n;
+  return PA_OK;
+}
+
+}  // namespace
+
+// ============================== C ABI ==============================
+struct pa_run {
+  Batch batch;
+  std::unique_ptr<RunHost> host;
+  double cap_sigma = 10.0;
+};
+
+struct pa_batch {
+  Batch batch;
+};
+
+extern "C" {
+
+const char* pa_last_error(void) { return g_err.c_str(); }
+int pa_version(void) { return 1; }
+
+int pa_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  HIPCHECK(hipGetDeviceProperties(&prop, dev));
+  if (name && name_len) snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+  return PA_OK;
+}
+
+int pa_schedule_steps(double Binit, double dB, double Bfin) {
+  if (!(dB > 0.0)) return -1;
+  return (int)schedule(Binit, dB, Bfin, 0).size() - 1;
+}
+
+size_t pa_run_footprint(const pa_params* p) {
+  RunHost h;
+  if (!p || plan_run(*p, h, 10.0)) return 0;
+  return layout(h, nullptr);
+}
+
+static int run_create_sigma(const pa_params* p, double sigma, pa_run** out) {
+  if (!p || !out) return fail(PA_ERR_INVALID, "null argument");
+  auto r = std::make_unique<pa_run>();
+  r->cap_sigma = sigma;
+  r->host = std::make_unique<RunHost>();
+  int rc = plan_run(*p, *r->host, sigma);
+  if (rc) return rc;
+  if ((rc = r->batch.create_stream())) return rc;
+  if ((rc = materialize(*r->host))) return rc;
+  r->batch.runs.push_back(r->host.get());
+  if ((rc = r->batch.build_tables())) return rc;
+  HIPCHECK(hipEventRecord(r->batch.ev0, r->batch.stream));
+  if ((rc = r->batch.launch_init())) return rc;
+  *out = r.release();
+  return PA_OK;
+}
+
+int pa_run_create(const pa_params* p, pa_run** out) { return run_create_sigma(p, 10.0, out); }
+
+int pa_run_step(pa_run* r, int nsteps) {
+  if (!r) return fail(PA_ERR_INVALID, "null run");
+  const int remaining = r->host->n_steps - r->batch.steps_launched;
+  const int k = nsteps < 0 ? remaining : std::min(nsteps, remaining);
+  r->host->results_valid = false;
+  return r->batch.run_steps(std::max(k, 0));
+}
+
+int pa_run_sync(pa_run* r) {
+  if (!r) return fail(PA_ERR_INVALID, "null run");
+  HIPCHECK(hipEventRecord(r->batch.ev1, r->batch.stream));
+  HIPCHECK(hipStreamSynchronize(r->batch.stream));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, r->batch.ev0, r->batch.ev1));
+  r->batch.total_ms = ms;
+  return fetch(*r->host, r->batch.stream);
+}
+
+int pa_run_steps_done(pa_run* r, int* done, int* total) {
+  if (!r) return fail(PA_ERR_INVALID, "null run");
+  int rc = pa_run_sync(r);
+  if (rc) return rc;
+  if (done) *done = r->host->st_h.step - 1;
+  if (total) *total = r->host->n_steps;
+  return PA_OK;
+}
+
+int pa_run_results(pa_run* r, pa_row* rows, int max_rows, int* nrows) {
+  if (!r || !rows || !nrows) return fail(PA_ERR_INVALID, "null argument");
+  if (!r->host->results_valid) {
+    int rc = pa_run_sync(r);
+    if (rc) return rc;
+  }
+  return rows_of(*r->host, rows, max_rows, nrows);
+}
+
+int pa_run_final_sequence(pa_run* r, uint64_t* is) {
+  if (!r || !is) return fail(PA_ERR_INVALID, "null argument");
+  if (!r->host->results_valid) {
+    int rc = pa_run_sync(r);
+    if (rc) return rc;
+  }
+  *is = r->host->st_h.IS;
+  return PA_OK;
+}
+
+int pa_run_population(pa_run* r, void* dst, size_t* bytes) {
+  if (!r || !bytes) return fail(PA_ERR_INVALID, "null argument");
+  int rc = pa_run_sync(r);
+  if (rc) return rc;
+  const RunHost& h = *r->host;
+  const int R = h.st_h.R;
+  const size_t unit = h.p.coding == PA_SSC ? (size_t)h.N : (size_t)h.N * (h.w / 8);
+  const size_t units = h.p.coding == PA_SSC ? (size_t)R : (size_t)ceil_div(R, h.w);
+  const size_t need = unit * units;
+  if (!dst) { *bytes = need; return PA_OK; }
+  if (*bytes < need) return fail(PA_ERR_INVALID, "buffer too small");
+  HIPCHECK(hipMemcpy(dst, h.desc.pop[h.st_h.buf], need, hipMemcpyDeviceToHost));
+  *bytes = need;
+  return PA_OK;
+}
+
+int pa_run_array(pa_run* r, int which, void* dst, size_t* bytes) {
+  if (!r || !bytes) return fail(PA_ERR_INVALID, "null argument");
+  int rc = pa_run_sync(r);
+  if (rc) return rc;
+  const RunHost& h = *r->host;
+  const int R = h.st_h.R;
+  const int units = h.p.coding == PA_SSC ? R : ceil_div(R, h.w);
+  const void* src = nullptr;
+  size_t need = 0;
+  const size_t members = h.p.coding == PA_SSC ? (size_t)R : (size_t)units * h.w;
+  switch (which) {
+    case 0: src = h.desc.IE; need = members * 4; break;
+    case 1: src = h.desc.M; need = members * 4; break;
+    case 2: case 3: {
+      // arrays of the last resampling: indexed by the parents (previous population)
+      const int Rp = h.st_h.step > 1 ? h.out_h[h.st_h.step - 1].Rprev : R;
+      const size_t pm = h.p.coding == PA_SSC ? (size_t)Rp : (size_t)ceil_div(Rp, h.w) * h.w;
+      src = which == 2 ? (const void*)h.desc.Roff : (const void*)h.desc.pre;
+      need = pm * 4;
+      break;
+    }
+    case 4: src = h.desc.amask[h.st_h.buf]; need = (size_t)units * 8; break;
+    default: return fail(PA_ERR_INVALID, "which");
+  }
+  if (!dst) { *bytes = need; return PA_OK; }
+  if (*bytes < need) return fail(PA_ERR_INVALID, "buffer too small");
+  HIPCHECK(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
+  *bytes = need;
+  return PA_OK;
+}
+
+int pa_run_timing(pa_run* r, double* sweep_ms, double* total_ms) {
+  if (!r) return fail(PA_ERR_INVALID, "null run");
+  if (sweep_ms) *sweep_ms = r->batch.sweep_ms;
+  if (total_ms) *total_ms = r->batch.total_ms;
+  return PA_OK;
+}
+
+int pa_run_set_timing(pa_run* r, int on) {
+  if (!r) return fail(PA_ERR_INVALID, "null run");
+  r->batch.timing = on != 0;
+  return PA_OK;
+}
+
+int pa_run_set_graph(pa_run* r, int on) {
+  if (!r) return fail(PA_ERR_INVALID, "null run");
+  r->batch.use_graph = on != 0;
+  return PA_OK;
+}
+
+int pa_run_destroy(pa_run* r) {
+  delete r;
+  return PA_OK;
+}
+
+int pa_run_all(const pa_params* p, pa_row* rows, int max_rows, int* nrows, uint64_t* final_is) {
+  double sigma = 10.0;
+  for (int attempt = 0; attempt < 4; ++attempt, sigma *= 4.0) {
+    pa_run* r = nullptr;
+    int rc = run_create_sigma(p, sigma, &r);
+    if (rc) return rc;
+    rc = pa_run_step(r, -1);
+    if (!rc) rc = pa_run_sync(r);
+    if (rc == PA_ERR_CAPACITY) { pa_run_destroy(r); continue; }
+    if (!rc) rc = rows_of(*r->host, rows, max_rows, nrows);
+    if (!rc && final_is) *final_is = r->host->st_h.IS;
+    pa_run_destroy(r);
+    return rc;
+  }
+  return fail(PA_ERR_CAPACITY, "population outgrew its buffers repeatedly");
+}
+
+// ---------------- batch ----------------
+int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out) {
+  if (!p || nruns < 1 || !out) return fail(PA_ERR_INVALID, "bad batch arguments");
+  auto b = std::make_unique<pa_batch>();
+  int rc;
+  if ((rc = b->batch.create_stream())) return rc;
+  for (int i = 0; i < nruns; ++i) {
+    auto h = std::make_unique<RunHost>();
+    if ((rc = plan_run(p[i], *h, 10.0))) return rc;
+    if ((rc = materialize(*h))) return rc;
+    b->batch.runs.push_back(h.get());
+    b->batch.owned.push_back(std::move(h));
+  }
+  b->batch.use_graph = use_graph != 0;
+  if ((rc = b->batch.build_tables())) return rc;
+  *out = b.release();
+  return PA_OK;
+}
+
+int pa_batch_run(pa_batch* b) {
+  if (!b) return fail(PA_ERR_INVALID, "null batch");
+  Batch& B = b->batch;
+  int rc;
+  HIPCHECK(hipEventRecord(B.ev0, B.stream));
+  if ((rc = B.launch_init())) return rc;
+  if ((rc = B.run_steps(B.max_steps))) return rc;
+  HIPCHECK(hipEventRecord(B.ev1, B.stream));
+  HIPCHECK(hipStreamSynchronize(B.stream));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, B.ev0, B.ev1));
+  B.total_ms = ms;
+  for (RunHost* h : B.runs)
+    if ((rc = fetch(*h, B.stream))) return rc;
+  return PA_OK;
+}
+
+int pa_batch_results(pa_batch* b, int run, pa_row* rows, int max_rows, int* nrows) {
+  if (!b || run < 0 || run >= (int)b->batch.runs.size()) return fail(PA_ERR_INVALID, "run index");
+  if (!b->batch.runs[run]->results_valid) return fail(PA_ERR_STATE, "batch not run");
+  return rows_of(*b->batch.runs[run], rows, max_rows, nrows);
+}
+
+int pa_batch_time(pa_batch* b, double* total_ms) {
+  if (!b) return fail(PA_ERR_INVALID, "null batch");
+  if (total_ms) *total_ms = b->batch.total_ms;
+  return PA_OK;
+}
+
+int pa_batch_destroy(pa_batch* b) {
+  delete b;
+  return PA_OK;
+}
+
+}  // extern "C"
+
+// ---------------- server: continuous batching under an HBM budget ----------------
+// Requests wait in a FIFO queue; at every tick the server retires finished runs, admits
+// queued runs while their footprint fits the budget (and the resident-run limit), and then
+// advances every resident run by the same number of steps with one batched launch
+// sequence.  Runs that join later simply start at their own step 1: every kernel reads its
+// run's own state, so a run's results do not depend on which other runs share the device.
+namespace {
+
+struct Request {
+  int64_t ticket = 0;
+  pa_params p{};
+  size_t footprint = 0;
+  std::unique_ptr<RunHost> h;
+  int state = 0;  // 0 queued, 1 resident, 2 done, 3 failed
+  int steps_left = 0;
+  int rc = PA_OK;
+  std::vector<pa_row> rows;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct pa_server {
+  size_t budget = 0;
+  int max_runs = 0;
+  size_t used = 0;
+  Batch batch;
+  std::deque<int64_t> queue;
+  std::map<int64_t, Request> reqs;
+  std::vector<int64_t> resident;
+  int64_t next_ticket = 1;
+  int completed = 0;
+  int chunk = 8;
+  bool dirty = false;
+};
+
+extern "C" {
+
+int pa_server_create(size_t hbm_budget_bytes, int max_runs, pa_server** out) {
+  if (!out || hbm_budget_bytes == 0 || max_runs < 1) return fail(PA_ERR_INVALID, "bad server arguments");
+  auto s = std::make_unique<pa_server>();
+  s->budget = hbm_budget_bytes;
+  s->max_runs = max_runs;
+  int rc = s->batch.create_stream();
+  if (rc) return rc;
+  s->batch.use_graph = true;
+  *out = s.release();
+  return PA_OK;
+}
+
+int pa_server_set_chunk(pa_server* s, int steps_per_tick, int use_graph) {
+  if (!s || steps_per_tick < 1) return fail(PA_ERR_INVALID, "bad chunk");
+  s->chunk = steps_per_tick;
+  s->batch.use_graph = use_graph != 0;
+  return PA_OK;
+}
+
+int pa_server_submit(pa_server* s, const pa_params* p, int64_t* ticket) {
+  if (!s || !p || !ticket) return fail(PA_ERR_INVALID, "null argument");
+  Request rq;
+  rq.ticket = s->next_ticket++;
+  rq.p = *p;
+  rq.h = std::make_unique<RunHost>();
+  int rc = plan_run(*p, *rq.h, 10.0);
+  if (rc) return rc;
+  rq.footprint = layout(*rq.h, nullptr);
+  if (rq.footprint > s->budget) return fail(PA_ERR_MEMORY, "request exceeds the HBM budget");
+  rq.t_submit = now_s();
+  *ticket = rq.ticket;
+  s->queue.push_back(rq.ticket);
+  s->reqs.emplace(rq.ticket, std::move(rq));
+  return PA_OK;
+}
+
+static int server_retire(pa_server* s) {
+  bool any = false;
+  for (int64_t t : s->resident) any |= s->reqs[t].steps_left == 0;
+  if (!any) return PA_OK;
+  HIPCHECK(hipStreamSynchronize(s->batch.stream));
+  const double tnow = now_s();
+  std::vector<int64_t> keep;
+  for (int64_t t : s->resident) {
+    Request& rq = s->reqs[t];
+    if (rq.steps_left > 0) { keep.push_back(t); continue; }
+    rq.rc = fetch(*rq.h, s->batch.stream);
+    if (rq.rc == PA_OK) {
+      rq.rows.resize(rq.h->n_steps + 1);
+      int n = 0;
+      rows_of(*rq.h, rq.rows.data(), (int)rq.rows.size(), &n);
+      rq.rows.resize(n);
+      rq.state = 2;
+    } else {
+      rq.state = 3;
+    }
+    rq.t_done = tnow;
+    s->used -= rq.footprint;
+    rq.h.reset();
+    s->completed++;
+    s->dirty = true;
+  }
+  s->resident.swap(keep);
+  return PA_OK;
+}
+
+static int server_admit(pa_server* s) {
+  std::vector<RunHost*> fresh;
+  while (!s->queue.empty() && (int)s->resident.size() < s->max_runs) {
+    Request& rq = s->reqs[s->queue.front()];
+    if (s->used + rq.footprint > s->budget) break;
+    int rc = materialize(*rq.h);
+    if (rc) return rc;
+    s->used += rq.footprint;
+    rq.state = 1;
+    rq.steps_left = rq.h->n_steps;
+    rq.t_admit = now_s();
+    s->resident.push_back(rq.ticket);
+    fresh.push_back(rq.h.get());
+    s->queue.pop_front();
+    s->dirty = true;
+  }
+  if (!fresh.empty()) {
+    // initial populations of the newcomers, on the server stream
+    Batch init;
+    init.stream = s->batch.stream;
+    init.runs = fresh;
+    int rc = init.build_tables();
+    if (!rc) rc = init.launch_init();
+    hipError_t e = hipStreamSynchronize(s->batch.stream);
+    init.stream = nullptr;  // borrowed
+    if (rc) return rc;
+    HIPCHECK(e);
+  }
+  return PA_OK;
+}
+
+int pa_server_tick(pa_server* s, int* completed) {
+  if (!s) return fail(PA_ERR_INVALID, "null server");
+  int rc;
+  if ((rc = server_retire(s))) return rc;
+  if ((rc = server_admit(s))) return rc;
+  if (s->dirty) {
+    s->batch.runs.clear();
+    for (int64_t t : s->resident) s->batch.runs.push_back(s->reqs[t].h.get());
+    if ((rc = s->batch.build_tables())) return rc;
+    s->dirty = false;
+  }
+  if (!s->resident.empty()) {
+    int k = s->chunk;
+    for (int64_t t : s->resident) k = std::min(k, s->reqs[t].steps_left);
+    if ((rc = s->batch.run_steps(k))) return rc;
+    for (int64_t t : s->resident) s->reqs[t].steps_left -= k;
+    if ((rc = server_retire(s))) return rc;
+  }
+  if (completed) *completed = s->completed;
+  return PA_OK;
+}
+
+int pa_server_poll(pa_server* s, int64_t ticket, int* done) {
+  if (!s || !done) return fail(PA_ERR_INVALID, "null argument");
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return fail(PA_ERR_INVALID, "unknown ticket");
+  *done = it->second.state >= 2 ? 1 : 0;
+  return it->second.state == 3 ? it->second.rc : PA_OK;
+}
+
+int pa_server_results(pa_server* s, int64_t ticket, pa_row* rows, int max_rows, int* nrows) {
+  if (!s || !rows || !nrows) return fail(PA_ERR_INVALID, "null argument");
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return fail(PA_ERR_INVALID, "unknown ticket");
+  if (it->second.state != 2) return fail(PA_ERR_STATE, "request not finished");
+  const int n = std::min((int)it->second.rows.size(), max_rows);
+  memcpy(rows, it->second.rows.data(), n * sizeof(pa_row));
+  *nrows = n;
+  return PA_OK;
+}
+
+int pa_server_latency(pa_server* s, int64_t ticket, double* submit_to_done_s, double* admit_to_done_s) {
+  if (!s) return fail(PA_ERR_INVALID, "null server");
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end() || it->second.state != 2) return fail(PA_ERR_STATE, "request not finished");
+  if (submit_to_done_s) *submit_to_done_s = it->second.t_done - it->second.t_submit;
+  if (admit_to_done_s) *admit_to_done_s = it->second.t_done - it->second.t_admit;
+  return PA_OK;
+}
+
+int pa_server_stats(pa_server* s, int* queued, int* resident, size_t* hbm_used) {
+  if (!s) return fail(PA_ERR_INVALID, "null server");
+  if (queued) *queued = (int)s->queue.size();
+  if (resident) *resident = (int)s->resident.size();
+  if (hbm_used) *hbm_used = s->used;
+  return PA_OK;
+}
+
+int pa_server_destroy(pa_server* s) {
+  if (!s) return PA_OK;
+  (void)hipStreamSynchronize(s->batch.stream);
+  s->batch.runs.clear();
+  delete s;
+  return PA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,959 @@
+// gfx950 kernels of the population-annealing engine.
+//
+// One temperature step of one or many runs is the launch sequence
+//   k_qpart -> k_tau -> k_scan -> k_resample -> k_sweep_{ssc,msc} -> k_avg -> k_fin
+// Every kernel maps blockIdx.x to (run, local block) through a prefix table, so the same
+// launches serve a single run (plain path) and a batch of independent runs.  All
+// population-size-dependent decisions read the device RunState, never the host.
+//
+// Reference counterparts (PAisingSSC.cu = SSC, PAisingMSC.cu = MSC):
+//   k_init_*   ReplicaInit + energyKer       SSC:150-157,195-215 / MSC:169-188,268-317
+//   k_qpart    QKer part 0 (and 1 for MSC)   SSC:217-224 / MSC:319-335
+//   k_tau      QKer final part, host lnQ, CalcTauKer   SSC:225-254,533 / MSC:336-360,696
+//   k_scan     CalcParSum                    SSC:256-270 / MSC:362-395
+//   k_resample resampleKer                   SSC:272-280 / MSC:397-435
+//   k_sweep_*  checkKerALL + energyKer       SSC:159-215 / MSC:190-317
+//   k_avg/k_fin CalcAverages + host thermodynamics bookkeeping  SSC:282-292,614-632
+
+#include <hip/hip_runtime.h>
+
+#include "pa_dev.h"
+#include "pa_prims.h"
+
+namespace pa {
+
+constexpr int kRB = kRedBlock;  // 1024
+
+__device__ __forceinline__ int find_run(const int32_t* __restrict__ off, int nruns, int b) {
+  int lo = 0, hi = nruns;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// ---- reference-order FP64 block sum (SSC:107-124) -----------------------------------
+// Warps of 32 lanes (two per wave64) reduce by shuffle-down 16,8,4,2,1; lane 0 of each warp
+// stores its sum, then warp 0 reduces the warp sums in the same tree.  Result in thread 0.
+__device__ __forceinline__ double warp32_tree(double v) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_down(v, off, 32);
+  return v;
+}
+
+__device__ double ref_block_sum(double v, double* sh) {
+  const int lane = threadIdx.x & 31, wid = threadIdx.x >> 5;
+  v = warp32_tree(v);
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 5;
+  v = (threadIdx.x < (unsigned)nw) ? sh[lane] : 0.0;
+  if (wid == 0) v = warp32_tree(v);
+  return v;
+}
+
+// Integer block sums (order-free).
+template <typename T>
+__device__ T block_sum_int(T v, T* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  return r;
+}
+
+// ---- geometry --------------------------------------------------------------------------
+// Sublattice site idx = ty*L/2 + tx.  Neighbours of a site of sublattice `side` in the other
+// sublattice: horizontal i1 (left or right by row parity), the same idx, row below i3 and
+// row above i4 (SSC:169-172 for A, SSC:181-184 for B).
+struct Site {
+  int idx, i1, i3, i4;
+};
+
+__device__ __forceinline__ Site site_of(int idx, int side, int ty, int tx, int L, int Ld2,
+                                        int Nh) {
+  Site s;
+  s.idx = idx;
+  const int odd = ty & 1;
+  const int right = (side == 0) ? odd : !odd;  // A: odd rows look right, B: even rows
+  const int x1 = right ? (tx + 1 == Ld2 ? 0 : tx + 1) : (tx == 0 ? Ld2 - 1 : tx - 1);
+  s.i1 = idx - tx + x1;
+  s.i3 = (ty == 0) ? idx + Nh - Ld2 : idx - Ld2;
+  s.i4 = (ty == L - 1) ? idx - Nh + Ld2 : idx + Ld2;
+  return s;
+}
+
+// ---- Boltzmann acceptance, SSC ----------------------------------------------------------
+// u < T  <=>  v < V(T) for the 32-bit word v behind u; V is computed on the host per step
+// for T = 1 (dE <= 0), exp(-4 beta), exp(-8 beta).
+__device__ __forceinline__ bool ssc_accept(int prod, uint32_t v, uint64_t V1, uint64_t V4,
+                                           uint64_t V8) {
+  const uint64_t thr = prod == 4 ? V8 : (prod == 2 ? V4 : V1);
+  return (uint64_t)v < thr;
+}
+
+// ---- MSC word update (MSC:207-229) -----------------------------------------------------
+// Bit-sliced count of antiparallel neighbours, one LCG value per replica bit, acceptance
+// mask: count >= 2 always, count 1 iff ran < T4, count 0 iff ran < T8.
+template <typename S>
+__device__ __forceinline__ S msc_acc(S s, S n1, S n2, S n3, S n4, S c4, S c8) {
+  const S I1 = s ^ n1, I2 = s ^ n2, I3 = s ^ n3, I4 = s ^ n4;
+  const S x12 = I1 ^ I2, x34 = I3 ^ I4, a12 = I1 & I2, a34 = I3 & I4;
+  const S sum0 = x12 ^ x34;
+  const S sum1 = (x12 & x34) ^ a12 ^ a34;
+  const S sum2 = a12 & a34;
+  const S s12 = sum1 | sum2;
+  return s12 | (~s12 & ((sum0 & c4) | (~sum0 & c8)));
+}
+
+// Condition masks for SW consecutive LCG values starting at `ran` (4 independent chains
+// interleaved for instruction-level parallelism).
+template <typename S, int NW>
+__device__ __forceinline__ void lcg_masks(uint32_t (&ran)[NW], uint32_t T4, uint32_t T8,
+                                          S (&c4)[NW], S (&c8)[NW]) {
+  constexpr int SW = (int)sizeof(S) * 8;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) { c4[j] = 0; c8[j] = 0; }
+#pragma unroll
+  for (int i = 0; i < SW; ++i) {
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      c4[j] |= (S)(ran[j] < T4) << i;
+      c8[j] |= (S)(ran[j] < T8) << i;
+      ran[j] = kLcgA * ran[j] + kLcgC;
+    }
+  }
+}
+
+// LCG jump: ran_j = A_j ran_0 + C_j (mod 2^32).
+__device__ __forceinline__ void lcg_jump(int j, uint32_t& Aj, uint32_t& Cj) {
+  uint32_t a = 1u, c = 0u;
+  for (int i = 0; i < j; ++i) { c = kLcgA * c + kLcgC; a = kLcgA * a; }
+  Aj = a;
+  Cj = c;
+}
+
+// ============================================================================================
+// Initial population + energies.  mode is implicit: launched once per run before step 1.
+// ============================================================================================
+
+// ReplicaInit SSC:150-157 (stream IS0 + t + 128 B; words 2k -> gA, 2k+1 -> gB; spin up iff
+// u >= 0.5) followed by energyKer SSC:195-215.
+__global__ __launch_bounds__(256) void k_init_ssc(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns,
+                                                  const uint64_t* __restrict__ is0) {
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  const int B = blockIdx.x - off[r];
+  if (B >= d.st->R) return;
+  extern __shared__ int8_t lat8[];
+  const int Nh = d.Nh, L = d.L, Ld2 = d.Ld2;
+  const uint64_t base = is0[r] + (uint64_t)B * kStreamsPerUnit;
+  for (int t = threadIdx.x; t < kStreamsPerUnit; t += blockDim.x) {
+    U4 blk{0, 0, 0, 0};
+    uint64_t cached = ~0ull;
+    int k = 0;
+    for (int idx = t; idx < Nh; idx += kStreamsPerUnit, ++k) {
+      for (int side = 0; side < 2; ++side) {
+        const uint64_t p = 2ull * k + side;
+        if ((p >> 2) != cached) { cached = p >> 2; blk = stream_block(d.seed, base + t, cached); }
+        const uint32_t v = u4_word(blk, (int)(p & 3));
+        lat8[side * Nh + idx] = uniform_double(v, d.umap) >= 0.5 ? 1 : -1;
+      }
+    }
+  }
+  __syncthreads();
+  int e = 0, m = 0;
+  for (int idx = threadIdx.x; idx < Nh; idx += blockDim.x) {
+    const int ty = idx / Ld2, tx = idx - ty * Ld2;
+    const int iL = ty * Ld2 + (tx == 0 ? Ld2 - 1 : tx - 1);
+    const int iU = (ty == L - 1 ? 0 : ty + 1) * Ld2 + tx;
+    const int sA = lat8[idx], sB = lat8[Nh + idx];
+    int Ai2, Bi2;
+    if (ty & 1) { Ai2 = sB; Bi2 = lat8[iL]; } else { Ai2 = lat8[Nh + iL]; Bi2 = sA; }
+    const int Ai4 = lat8[Nh + iU], Bi4 = lat8[iU];
+    e += -sA * (Ai2 + Ai4) - sB * (Bi2 + Bi4);
+    m += sA + sB;
+  }
+  __shared__ int shi[16];
+  const int E = block_sum_int<int>(e, shi);
+  const int Mg = block_sum_int<int>(m, shi);
+  if (threadIdx.x == 0) { d.IE[B] = E; d.M[B] = Mg; }
+  int8_t* g = (int8_t*)d.pop[0] + (size_t)B * d.N;
+  for (int i = threadIdx.x; i < d.N; i += blockDim.x) g[i] = lat8[i];
+}
+
+// Per-lane energy and magnetisation of an MSC lattice slice held in LDS (MSC:268-317):
+// per site pair e = 2 (#antiparallel of 4 bonds) - 4, m = 2 (bA + bB) - 2, summed per lane.
+template <typename S>
+__device__ void msc_energy_slice(const S* lat, int L, int Ld2, int Nh, int* accE, int* accM,
+                                 int lane0, int32_t* IE, int32_t* Mo, size_t lane_base) {
+  constexpr int SW = (int)sizeof(S) * 8;
+  for (int i = threadIdx.x; i < SW; i += blockDim.x) { accE[i] = 0; accM[i] = 0; }
+  __syncthreads();
+  const int nq = blockDim.x / SW > 0 ? blockDim.x / SW : 1;
+  const int lane = threadIdx.x % SW, q = threadIdx.x / SW;
+  if (q < nq) {
+    int e = 0, m = 0;
+    for (int idx = q; idx < Nh; idx += nq) {
+      const int ty = idx / Ld2, tx = idx - ty * Ld2;
+      const int iL = ty * Ld2 + (tx == 0 ? Ld2 - 1 : tx - 1);
+      const int iU = (ty == L - 1 ? 0 : ty + 1) * Ld2 + tx;
+      const S sA = lat[idx], sB = lat[Nh + idx];
+      S Ai2, Bi2;
+      if (ty & 1) { Ai2 = sB; Bi2 = lat[iL]; } else { Ai2 = lat[Nh + iL]; Bi2 = sA; }
+      const S Ai4 = lat[Nh + iU], Bi4 = lat[iU];
+      const S I1 = sA ^ Ai2, I2 = sA ^ Ai4, I3 = sB ^ Bi2, I4 = sB ^ Bi4;
+      const int c = (int)((I1 >> lane) & 1) + (int)((I2 >> lane) & 1) + (int)((I3 >> lane) & 1) +
+                    (int)((I4 >> lane) & 1);
+      e += 2 * c - 4;
+      m += 2 * ((int)((sA >> lane) & 1) + (int)((sB >> lane) & 1)) - 2;
+    }
+    atomicAdd(&accE[lane], e);
+    atomicAdd(&accM[lane], m);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < SW; i += blockDim.x) {
+    IE[lane_base + lane0 + i] = accE[i];
+    Mo[lane_base + lane0 + i] = accM[i];
+  }
+}
+
+// ReplicaInit MSC:169-188 (w=32: words 2k, 2k+1; w=64: (word 4k << 32) + word 4k+1 and
+// (word 4k+2 << 32) + word 4k+3, first draw in the high half), written straight to HBM;
+// the energies follow from a zero-sweep pass of k_sweep_msc.
+template <typename W>
+__global__ __launch_bounds__(128) void k_init_msc(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns,
+                                                  const uint64_t* __restrict__ is0) {
+  constexpr int w = (int)sizeof(W) * 8;
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  const int B = blockIdx.x - off[r];
+  const int R0 = d.st->R;
+  if (B >= ceil_div(R0, w)) return;
+  const int Nh = d.Nh;
+  W* gl = (W*)d.pop[0] + (size_t)B * d.N;
+  const uint64_t base = is0[r] + (uint64_t)B * kStreamsPerUnit;
+  constexpr int per = (w == 32) ? 1 : 2;  // stream words per lattice word
+  const int t = threadIdx.x;
+  U4 blk{0, 0, 0, 0};
+  uint64_t cached = ~0ull;
+  int k = 0;
+  for (int idx = t; idx < Nh; idx += kStreamsPerUnit, ++k) {
+    for (int side = 0; side < 2; ++side) {
+      W val = 0;
+      for (int h = 0; h < per; ++h) {
+        const uint64_t p = (uint64_t)(2 * k + side) * per + h;
+        if ((p >> 2) != cached) { cached = p >> 2; blk = stream_block(d.seed, base + t, cached); }
+        val = (per == 1) ? (W)u4_word(blk, (int)(p & 3))
+                         : (W)(((uint64_t)val << 32) + u4_word(blk, (int)(p & 3)));
+      }
+      gl[side * Nh + idx] = val;
+    }
+  }
+  if (t == 0) {
+    uint64_t mask = 0;
+    for (int i = 0; i < w; ++i)
+      if ((int64_t)B * w + i < R0) mask |= 1ull << i;
+    d.amask[0][B] = mask;
+  }
+}
+
+// ============================================================================================
+// Q partial sums: exp(-dB (E - Emean)) summed in the reference tree order.
+// SSC (QKer part 0, SSC:220-224): one block per 1024 replicas.
+// MSC (QKer parts 0+1, MSC:321-335): per group a w-thread block sum over lanes, then one
+// block per 1024 groups.  One thread per group here evaluates its group's lane tree.
+// ============================================================================================
+__global__ __launch_bounds__(1024) void k_qpart(const RunDesc* __restrict__ runs,
+                                                const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKGroupChunks * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int c = blockIdx.x - off[r];
+  const int R = st->R, step = st->step;
+  const double dB = d.dbeta[step], Emean = st->Emean;
+  __shared__ double sh[32];
+  double f = 0.0;
+  if (d.coding == kSSC) {
+    if (c >= ceil_div(R, kRB)) return;
+    const int idx = c * kRB + threadIdx.x;
+    f = (idx < R) ? exp_fp64(-dB * ((double)d.IE[idx] - Emean)) : 0.0;
+  } else {
+    const int w = d.w, rg = ceil_div(R, w);
+    if (c >= ceil_div(rg, kRB)) return;
+    const int g = c * kRB + threadIdx.x;
+    if (g < rg) {
+      const uint64_t mask = d.amask[st->buf][g];
+      double hsum[2];
+      for (int h = 0; h < w / 32; ++h) {
+        double v[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+          const int br = 32 * h + i;
+          v[i] = ((mask >> br) & 1) ? exp_fp64(-dB * ((double)d.IE[(size_t)g * w + br] - Emean))
+                                    : 0.0;
+        }
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1)
+#pragma unroll
+          for (int i = 0; i < o; ++i) v[i] = v[i] + v[i + o];
+        hsum[h] = v[0];
+      }
+      f = (w == 32) ? hsum[0] : hsum[0] + hsum[1];
+    }
+  }
+  const double s = ref_block_sum(f, sh);
+  if (threadIdx.x == 0) {
+    d.qblk[c] = s;
+    if (c == 0) {
+      StepOut& o = d.out[step];
+      o.roff_hash = 0;
+      o.spin_hash = 0;
+    }
+  }
+}
+
+// ============================================================================================
+// k_tau: q from the block partials (QKer last part, SSC:225-233 / MSC:336-344), lnQ
+// (SSC:533), then copy counts (CalcTauKer, SSC:236-254 / MSC:347-360):
+//   mu = Rinit/R exp(-dB E - lnQ), Roff = floor(mu) + [u < mu - floor(mu)],
+//   u from word 0 of subsequence IS + n (n = replica, or group*w + lane for MSC).
+// One block per 1024 population members; every block evaluates q itself (identical order).
+// ============================================================================================
+__global__ __launch_bounds__(1024) void k_tau(const RunDesc* __restrict__ runs,
+                                              const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKChunks * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int c = blockIdx.x - off[r];
+  const int R = st->R, step = st->step;
+  const int w = d.w;
+  const int nitems = (d.coding == kSSC) ? R : ceil_div(R, w);  // QKer's reduction items
+  const int nel = (d.coding == kSSC) ? R : ceil_div(R, w) * w;  // copy-count slots
+  if (c >= ceil_div(nel, kRB)) return;
+  __shared__ double sh[32];
+  __shared__ uint32_t shu[16];
+  __shared__ uint64_t shh[16];
+  const int nqb = ceil_div(nitems, kRB);
+  double acc = 0.0;
+  for (int j = 0; j * kRB < nitems; j += kRB) {
+    const int b = threadIdx.x + j;
+    const double f = (b < nqb) ? d.qblk[b] : 0.0;
+    acc += ref_block_sum(f, sh);
+    __syncthreads();
+  }
+  __shared__ double lnq_sh;
+  if (threadIdx.x == 0) {
+    const double dB = d.dbeta[step];
+    const double lnQ = -(dB)*st->Emean + log_fp64(acc) - log_fp64((double)R);
+    lnq_sh = lnQ;
+    if (c == 0) {
+      d.out[step].q = acc;
+      d.out[step].lnQ = lnQ;
+    }
+  }
+  __syncthreads();
+  const double lnQ = lnq_sh;
+  const double dB = d.dbeta[step];
+  const double ratio = (double)d.Rinit / (double)R;
+  const int n = c * kRB + threadIdx.x;
+  uint32_t roff = 0;
+  uint64_t hv = 0;
+  if (n < nel) {
+    bool act = true;
+    if (d.coding == kMSC) act = (d.amask[st->buf][n / w] >> (n % w)) & 1;
+    if (act) {
+      const double mu = ratio * exp_fp64(fma_(-dB, (double)d.IE[n], -lnQ));
+      const double mf = floor(mu);
+      const uint32_t v = stream_block(d.seed, st->IS + (uint64_t)n, 0).x;
+      roff = (uniform_double(v, d.umap) < (mu - mf)) ? (uint32_t)(mf + 1.0) : (uint32_t)mf;
+    }
+    d.Roff[n] = roff;
+    if (d.trace) hv = mix64(mix64((uint64_t)n) ^ (uint64_t)roff);
+  }
+  const uint32_t tot = block_sum_int<uint32_t>(roff, shu);
+  if (threadIdx.x == 0) d.cblk[c] = tot;
+  if (d.trace) {
+    const uint64_t ht = block_sum_int<uint64_t>(hv, shh);
+    if (threadIdx.x == 0) atomicAdd((unsigned long long*)&d.out[step].roff_hash, (unsigned long long)ht);
+  }
+}
+
+// ============================================================================================
+// k_scan: exclusive prefix of Roff in population order (CalcParSum), R_i, capacity check.
+// ============================================================================================
+__global__ __launch_bounds__(1024) void k_scan(const RunDesc* __restrict__ runs,
+                                               const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKChunks * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int c = blockIdx.x - off[r];
+  const int R = st->R, w = d.w;
+  const int nel = (d.coding == kSSC) ? R : ceil_div(R, w) * w;
+  const int nch = ceil_div(nel, kRB);
+  if (c >= nch) return;
+  __shared__ uint32_t shu[16];
+  __shared__ uint32_t wsum[16];
+  // offset of this chunk: sum of the preceding chunk totals
+  uint32_t part = 0;
+  for (int b = threadIdx.x; b < c; b += blockDim.x) part += d.cblk[b];
+  const uint32_t base = block_sum_int<uint32_t>(part, shu);
+  __shared__ uint32_t base_sh;
+  if (threadIdx.x == 0) base_sh = base;
+  __syncthreads();
+  const int n = c * kRB + threadIdx.x;
+  const uint32_t v = (n < nel) ? d.Roff[n] : 0u;
+  // inclusive wave scan
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  uint32_t wpre = 0;
+  for (int i = 0; i < wv; ++i) wpre += wsum[i];
+  const uint32_t excl = base_sh + wpre + x - v;
+  if (n < nel) d.pre[n] = excl;
+  if (c == nch - 1 && threadIdx.x == blockDim.x - 1) {
+    const uint32_t Rn = excl + v;
+    st->Rnew = (int)Rn;
+    const int units_needed = (d.coding == kSSC) ? (int)Rn : ceil_div((int)Rn, w);
+    if (Rn == 0 || units_needed > d.cap_units) st->overflow = 1;
+  }
+}
+
+// Parent of child j: the last parent p with pre[p] <= j (pre is non-decreasing, the true
+// parent satisfies pre[p] <= j < pre[p] + Roff[p]).
+__device__ __forceinline__ int parent_of(const uint32_t* __restrict__ pre, int n, uint32_t j) {
+  int lo = 0, hi = n;  // pre[lo] <= j always (pre[0] = 0)
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= j) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ============================================================================================
+// k_resample.  SSC (resampleKer SSC:272-280): child j receives its parent's N spins
+// (children of a parent are contiguous and in parent order).  All N sites are copied.
+// MSC (MSC:397-435): child n = prefix + p lives in new group n mod rg_new, lane n div rg_new;
+// each new word gathers one bit from each lane's parent (same site).
+// One block per unit of the new population.
+// ============================================================================================
+__global__ __launch_bounds__(256) void k_resample(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int j = blockIdx.x - off[r];
+  const int R = st->R, Rn = st->Rnew, w = d.w, buf = st->buf;
+  if (d.coding == kSSC) {
+    if (j >= Rn) return;
+    __shared__ int par_sh;
+    if (threadIdx.x == 0) par_sh = parent_of(d.pre, R, (uint32_t)j);
+    __syncthreads();
+    const int p = par_sh;
+    const uint8_t* src = (const uint8_t*)d.pop[buf] + (size_t)p * d.N;
+    uint8_t* dst = (uint8_t*)d.pop[buf ^ 1] + (size_t)j * d.N;
+    if ((d.N & 15) == 0) {
+      const uint4* s4 = (const uint4*)src;
+      uint4* d4 = (uint4*)dst;
+      for (int i = threadIdx.x; i < d.N / 16; i += blockDim.x) d4[i] = s4[i];
+    } else {
+      for (int i = threadIdx.x; i < d.N; i += blockDim.x) dst[i] = src[i];
+    }
+    return;
+  }
+  const int rgn = ceil_div(Rn, w);
+  if (j >= rgn) return;
+  const int nold = ceil_div(R, w) * w;
+  __shared__ int pg[64], pb[64];
+  if (threadIdx.x < (unsigned)w) {
+    const int64_t n = (int64_t)threadIdx.x * rgn + j;
+    if (n < Rn) {
+      const int p = parent_of(d.pre, nold, (uint32_t)n);
+      pg[threadIdx.x] = p / w;
+      pb[threadIdx.x] = p % w;
+    } else {
+      pg[threadIdx.x] = -1;
+      pb[threadIdx.x] = 0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t m = 0;
+    for (int k = 0; k < w; ++k)
+      if (pg[k] >= 0) m |= 1ull << k;
+    d.amask[buf ^ 1][j] = m;
+  }
+  __syncthreads();
+  const int N = d.N;
+  if (w == 32) {
+    const uint32_t* src = (const uint32_t*)d.pop[buf];
+    uint32_t* dst = (uint32_t*)d.pop[buf ^ 1] + (size_t)j * N;
+    for (int s = threadIdx.x; s < N; s += blockDim.x) {
+      uint32_t acc = 0;
+      for (int k = 0; k < 32; ++k) {
+        const int g = pg[k];
+        if (g >= 0) acc |= ((src[(size_t)g * N + s] >> pb[k]) & 1u) << k;
+      }
+      dst[s] = acc;
+    }
+  } else {
+    const uint64_t* src = (const uint64_t*)d.pop[buf];
+    uint64_t* dst = (uint64_t*)d.pop[buf ^ 1] + (size_t)j * N;
+    for (int s = threadIdx.x; s < N; s += blockDim.x) {
+      uint64_t acc = 0;
+      for (int k = 0; k < 64; ++k) {
+        const int g = pg[k];
+        if (g >= 0) acc |= ((src[(size_t)g * N + s] >> pb[k]) & 1ull) << k;
+      }
+      dst[s] = acc;
+    }
+  }
+}
+
+// ============================================================================================
+// Sweeps.  The unit's lattice lives in LDS for all EqSw sweeps of the step.  Site (side h,
+// idx) of sweep s is driven by stream t = idx mod 128 of the unit (subsequence
+// IS_sweep + t + 128*unit), word p = (2s + h) n_t + idx div 128 with n_t = ceil((N/2-t)/128)
+// (SSC:164-188, MSC:195-261).  When N/2 is a multiple of 512 every stream owns whole Philox
+// blocks per half sweep, and each thread evaluates one block for four sites at once.
+// ============================================================================================
+
+// SSC: one block per replica.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int B = blockIdx.x - off[r];
+  const int Rn = st->Rnew;
+  if (B >= Rn) return;
+  extern __shared__ int8_t lat8[];
+  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
+  const int step = st->step;
+  int8_t* g = (int8_t*)d.pop[st->buf ^ 1] + (size_t)B * N;
+  for (int i = threadIdx.x; i < N; i += NT) lat8[i] = g[i];
+  const uint64_t V1 = d.thr[3 * step + 0], V4 = d.thr[3 * step + 1], V8 = d.thr[3 * step + 2];
+  const uint64_t sub0 = st->IS + (uint64_t)st->R + (uint64_t)B * kStreamsPerUnit;
+  const uint64_t seed = d.seed;
+  const uint32_t nsw = d.eqsweeps;
+  __syncthreads();
+  if ((Nh & 511) == 0) {
+    // aligned: thread = (stream t, part); blocks m = part, part + P, ... of nb per half
+    constexpr int P = NT / kStreamsPerUnit;
+    const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
+    const int nb = Nh / 512;
+    for (uint32_t s = 0; s < nsw; ++s) {
+      for (int h = 0; h < 2; ++h) {
+        int8_t* own = lat8 + h * Nh;
+        const int8_t* oth = lat8 + (h ^ 1) * Nh;
+        for (int m = part; m < nb; m += P) {
+          const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+          const uint32_t v[4] = {rb.x, rb.y, rb.z, rb.w};
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const int idx = t + kStreamsPerUnit * (4 * m + jj);
+            const int ty = idx / Ld2, tx = idx - ty * Ld2;
+            const Site q = site_of(idx, h, ty, tx, L, Ld2, Nh);
+            const int sp = own[idx];
+            const int prod = sp * (oth[q.i1] + oth[idx] + oth[q.i3] + oth[q.i4]);
+            if (ssc_accept(prod, v[jj], V1, V4, V8)) own[idx] = (int8_t)-sp;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  } else {
+    // generic: one thread per stream, sites in stream order with a cached Philox block
+    const int t = threadIdx.x;
+    const bool has = t < kStreamsPerUnit && t < Nh;
+    const int nt = has ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
+    U4 rb{0, 0, 0, 0};
+    uint64_t cached = ~0ull;
+    for (uint32_t s = 0; s < nsw; ++s) {
+      for (int h = 0; h < 2; ++h) {
+        int8_t* own = lat8 + h * Nh;
+        const int8_t* oth = lat8 + (h ^ 1) * Nh;
+        for (int k = 0; k < nt; ++k) {
+          const uint64_t p = (uint64_t)(2 * s + h) * nt + k;
+          if ((p >> 2) != cached) { cached = p >> 2; rb = stream_block(seed, sub0 + t, cached); }
+          const uint32_t v = u4_word(rb, (int)(p & 3));
+          const int idx = t + kStreamsPerUnit * k;
+          const int ty = idx / Ld2, tx = idx - ty * Ld2;
+          const Site q = site_of(idx, h, ty, tx, L, Ld2, Nh);
+          const int sp = own[idx];
+          const int prod = sp * (oth[q.i1] + oth[idx] + oth[q.i3] + oth[q.i4]);
+          if (ssc_accept(prod, v, V1, V4, V8)) own[idx] = (int8_t)-sp;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // energy and magnetisation (energyKer SSC:195-215)
+  int e = 0, mm = 0;
+  for (int idx = threadIdx.x; idx < Nh; idx += NT) {
+    const int ty = idx / Ld2, tx = idx - ty * Ld2;
+    const int iL = ty * Ld2 + (tx == 0 ? Ld2 - 1 : tx - 1);
+    const int iU = (ty == L - 1 ? 0 : ty + 1) * Ld2 + tx;
+    const int sA = lat8[idx], sB = lat8[Nh + idx];
+    int Ai2, Bi2;
+    if (ty & 1) { Ai2 = sB; Bi2 = lat8[iL]; } else { Ai2 = lat8[Nh + iL]; Bi2 = sA; }
+    const int Ai4 = lat8[Nh + iU], Bi4 = lat8[iU];
+    e += -sA * (Ai2 + Ai4) - sB * (Bi2 + Bi4);
+    mm += sA + sB;
+  }
+  __shared__ int shi[16];
+  const int E = block_sum_int<int>(e, shi);
+  const int Mg = block_sum_int<int>(mm, shi);
+  if (threadIdx.x == 0) { d.IE[B] = E; d.M[B] = Mg; }
+  for (int i = threadIdx.x; i < N; i += NT) g[i] = lat8[i];
+}
+
+// MSC: one block per (group, slice).  A slice holds SW = 8*sizeof(S) consecutive replica
+// bits of every site word; its LCG chain starts at ran_{q*SW} = A_{q SW} ran_0 + C_{q SW}.
+// Slices let lattices too large for LDS (L=256 x 64 bits = 512 KiB) and populations with
+// few groups use the whole chip; they do not change any stream.
+template <typename W, typename S, int NT, bool INIT>
+__global__ __launch_bounds__(NT) void k_sweep_msc(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns) {
+  constexpr int w = (int)sizeof(W) * 8;
+  constexpr int SW = (int)sizeof(S) * 8;
+  constexpr int NS = w / SW;  // slices per word
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int bu = blockIdx.x / NS;
+  const int qs = blockIdx.x % NS;
+  const int r = find_run(off, nruns, bu);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!INIT && (!st->active || st->overflow)) return;
+  const int g = bu - off[r];
+  const int Rn = INIT ? st->R : st->Rnew;
+  const int rgn = ceil_div(Rn, w);
+  if (g >= rgn) return;
+  extern __shared__ __align__(16) uint8_t smem_sw[];
+  S* lat = (S*)smem_sw;
+  __shared__ int accE[SW], accM[SW];
+  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
+  const int step = st->step;
+  uint8_t* gbase = (uint8_t*)d.pop[INIT ? 0 : st->buf ^ 1] + (size_t)g * N * sizeof(W) +
+                   qs * sizeof(S);
+  for (int i = threadIdx.x; i < N; i += NT) lat[i] = *(const S*)(gbase + (size_t)i * sizeof(W));
+  const uint32_t T4 = INIT ? 0u : (uint32_t)d.thr[3 * step + 0];
+  const uint32_t T8 = INIT ? 0u : (uint32_t)d.thr[3 * step + 1];
+  const uint64_t sub0 =
+      st->IS + (uint64_t)ceil_div(st->R, w) * w + (uint64_t)g * kStreamsPerUnit;
+  const uint64_t seed = d.seed;
+  const uint32_t nsw = INIT ? 0u : d.eqsweeps;
+  uint32_t JA = 1u, JC = 0u;
+  lcg_jump(qs * SW, JA, JC);
+  __syncthreads();
+  if ((Nh & 511) == 0) {
+    constexpr int P = NT / kStreamsPerUnit;
+    const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
+    const int nb = Nh / 512;
+    for (uint32_t s = 0; s < nsw; ++s) {
+      for (int h = 0; h < 2; ++h) {
+        S* own = lat + h * Nh;
+        const S* oth = lat + (h ^ 1) * Nh;
+        for (int m = part; m < nb; m += P) {
+          const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+          uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
+          S c4[4], c8[4];
+          lcg_masks<S, 4>(ran, T4, T8, c4, c8);
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const int idx = t + kStreamsPerUnit * (4 * m + jj);
+            const int ty = idx / Ld2, tx = idx - ty * Ld2;
+            const Site q = site_of(idx, h, ty, tx, L, Ld2, Nh);
+            const S sp = own[idx];
+            own[idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[idx], oth[q.i3], oth[q.i4], c4[jj], c8[jj]);
+          }
+        }
+        __syncthreads();
+      }
+    }
+  } else {
+    const int t = threadIdx.x;
+    const bool has = t < kStreamsPerUnit && t < Nh;
+    const int nt = has ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
+    U4 rb{0, 0, 0, 0};
+    uint64_t cached = ~0ull;
+    for (uint32_t s = 0; s < nsw; ++s) {
+      for (int h = 0; h < 2; ++h) {
+        S* own = lat + h * Nh;
+        const S* oth = lat + (h ^ 1) * Nh;
+        for (int k = 0; k < nt; ++k) {
+          const uint64_t p = (uint64_t)(2 * s + h) * nt + k;
+          if ((p >> 2) != cached) { cached = p >> 2; rb = stream_block(seed, sub0 + t, cached); }
+          uint32_t ran[1] = {JA * u4_word(rb, (int)(p & 3)) + JC};
+          S c4[1], c8[1];
+          lcg_masks<S, 1>(ran, T4, T8, c4, c8);
+          const int idx = t + kStreamsPerUnit * k;
+          const int ty = idx / Ld2, tx = idx - ty * Ld2;
+          const Site q = site_of(idx, h, ty, tx, L, Ld2, Nh);
+          const S sp = own[idx];
+          own[idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[idx], oth[q.i3], oth[q.i4], c4[0], c8[0]);
+        }
+        __syncthreads();
+      }
+    }
+  }
+  msc_energy_slice<S>(lat, L, Ld2, Nh, accE, accM, qs * SW, d.IE, d.M, (size_t)g * w);
+  if (!INIT)
+    for (int i = threadIdx.x; i < N; i += NT) *(S*)(gbase + (size_t)i * sizeof(W)) = lat[i];
+}
+
+// ============================================================================================
+// Averages (CalcAverages SSC:282-292 / MSC:437-450): block tree sums of E, E^2, |M|, M^2,
+// M^4 over 1024 replicas (SSC) or over 1024 groups for one lane (MSC: block = (chunk, lane)),
+// plus integer sums.  Block partials are combined in launch order by k_fin.
+// ============================================================================================
+__global__ __launch_bounds__(1024) void k_avg(const RunDesc* __restrict__ runs,
+                                              const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKAvg * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int c = blockIdx.x - off[r];
+  const int Rn = st->Rnew, w = d.w, buf = st->buf ^ 1;
+  int nb, b, by;
+  if (d.coding == kSSC) {
+    nb = ceil_div(Rn, kRB);
+    if (c >= nb) return;
+    b = c; by = 0;
+  } else {
+    nb = ceil_div(ceil_div(Rn, w), kRB);
+    if (c >= nb * w) return;
+    by = c / nb; b = c % nb;
+  }
+  __shared__ double sh[32];
+  __shared__ int64_t shl[16];
+  const int idx = b * kRB + threadIdx.x;
+  double e = 0.0, m = 0.0;
+  int64_t ei = 0, mi = 0;
+  if (d.coding == kSSC) {
+    if (idx < Rn) { e = d.IE[idx]; m = d.M[idx]; ei = d.IE[idx]; mi = d.M[idx]; }
+  } else {
+    const int rgn = ceil_div(Rn, w);
+    if (idx < rgn && ((d.amask[buf][idx] >> by) & 1)) {
+      const size_t k = (size_t)idx * w + by;
+      e = d.IE[k]; m = d.M[k]; ei = d.IE[k]; mi = d.M[k];
+    }
+  }
+  if (m < 0) m = -m;
+  const double e2 = e * e, m2 = m * m, m4 = m2 * m2;
+  double* o = d.avblk + (size_t)c * 5;
+  double s;
+  s = ref_block_sum(e, sh);  if (threadIdx.x == 0) o[0] = s;  __syncthreads();
+  s = ref_block_sum(e2, sh); if (threadIdx.x == 0) o[1] = s;  __syncthreads();
+  s = ref_block_sum(m, sh);  if (threadIdx.x == 0) o[2] = s;  __syncthreads();
+  s = ref_block_sum(m2, sh); if (threadIdx.x == 0) o[3] = s;  __syncthreads();
+  s = ref_block_sum(m4, sh); if (threadIdx.x == 0) o[4] = s;  __syncthreads();
+  const int64_t se = block_sum_int<int64_t>(ei, shl);
+  const int64_t sm = block_sum_int<int64_t>(mi, shl);
+  if (threadIdx.x == 0) { d.iblk[2 * c] = se; d.iblk[2 * c + 1] = sm; }
+}
+
+// Spin-configuration hash of the new population (trace mode only): sum over 8-byte chunks
+// i of mix64(mix64(i) ^ chunk_i), chunks numbered unit-major, zero-padded per unit.
+__global__ __launch_bounds__(256) void k_hash(const RunDesc* __restrict__ runs,
+                                              const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!d.trace || !st->active || st->overflow) return;
+  const int u = blockIdx.x - off[r];
+  const int Rn = st->Rnew;
+  const int units = (d.coding == kSSC) ? Rn : ceil_div(Rn, d.w);
+  if (u >= units) return;
+  const size_t bytes = (d.coding == kSSC) ? (size_t)d.N : (size_t)d.N * (d.w / 8);
+  const size_t chunks = (bytes + 7) / 8;
+  const uint8_t* p = (const uint8_t*)d.pop[st->buf ^ 1] + (size_t)u * bytes;
+  uint64_t h = 0;
+  for (size_t c = threadIdx.x; c < chunks; c += blockDim.x) {
+    uint64_t v = 0;
+    const size_t nbt = bytes - c * 8 < 8 ? bytes - c * 8 : 8;
+    for (size_t i = 0; i < nbt; ++i) v |= (uint64_t)p[c * 8 + i] << (8 * i);
+    h += mix64(mix64((uint64_t)u * chunks + c) ^ v);
+  }
+  __shared__ uint64_t shh[4];
+  const uint64_t t = block_sum_int<uint64_t>(h, shh);
+  if (threadIdx.x == 0)
+    atomicAdd((unsigned long long*)&d.out[st->step].spin_hash, (unsigned long long)t);
+}
+
+// Step bookkeeping (SSC:620-632 device part): combine the average partials in launch order,
+// write the step record, advance R, IS (tau draws + sweep streams), Emean and the step.
+__global__ void k_fin(const RunDesc* __restrict__ runs, int nruns) {
+  const int r = blockIdx.x;
+  if (r >= nruns || threadIdx.x != 0) return;
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int step = st->step, R = st->R, Rn = st->Rnew, w = d.w;
+  StepOut& o = d.out[step];
+  double sums[5] = {0, 0, 0, 0, 0};
+  int64_t se = 0, sm = 0;
+  int nblk;
+  if (d.coding == kSSC) nblk = ceil_div(Rn, kRB);
+  else nblk = ceil_div(ceil_div(Rn, w), kRB) * w;
+  for (int c = 0; c < nblk; ++c) {
+    for (int q = 0; q < 5; ++q) sums[q] += d.avblk[(size_t)c * 5 + q];
+    se += d.iblk[2 * c];
+    sm += d.iblk[2 * c + 1];
+  }
+  for (int q = 0; q < 5; ++q) o.sums[q] = sums[q];
+  o.sumE = se;
+  o.sumM = sm;
+  o.R = Rn;
+  o.Rprev = R;
+  uint64_t IS = st->IS;
+  if (d.coding == kSSC) IS += (uint64_t)R + (uint64_t)Rn * kStreamsPerUnit;
+  else IS += (uint64_t)ceil_div(R, w) * w + (uint64_t)ceil_div(Rn, w) * kStreamsPerUnit;
+  o.IS_after = IS;
+  st->IS = IS;
+  st->Emean = sums[0] / (double)Rn;
+  st->R = Rn;
+  st->buf ^= 1;
+  st->step = step + 1;
+  if (step + 1 > d.n_steps) st->active = 0;
+}
+
+}  // namespace pa
+
+// ============================================================================================
+// Launch entry points used by the host engine (one per kernel, hipLaunchKernelGGL on the
+// caller's stream).
+// ============================================================================================
+namespace pa {
+
+template <bool INIT>
+static hipError_t launch_sweep_msc(int w, int sw, int nt, int nblocks, size_t lds,
+                                   hipStream_t s, const RunDesc* runs, const int32_t* offs,
+                                   int nruns) {
+#define PA_MSC(WT, ST, NTV)                                                                    \
+  hipLaunchKernelGGL((k_sweep_msc<WT, ST, NTV, INIT>), dim3(nblocks), dim3(NTV), lds, s, runs, \
+                     offs, nruns)
+  if (w == 32) {
+    if (sw == 32) { if (nt == 256) PA_MSC(uint32_t, uint32_t, 256); else PA_MSC(uint32_t, uint32_t, 128); }
+    else if (sw == 16) { if (nt == 256) PA_MSC(uint32_t, uint16_t, 256); else PA_MSC(uint32_t, uint16_t, 128); }
+    else { if (nt == 256) PA_MSC(uint32_t, uint8_t, 256); else PA_MSC(uint32_t, uint8_t, 128); }
+  } else {
+    if (sw == 64) { if (nt == 256) PA_MSC(uint64_t, uint64_t, 256); else PA_MSC(uint64_t, uint64_t, 128); }
+    else if (sw == 32) { if (nt == 256) PA_MSC(uint64_t, uint32_t, 256); else PA_MSC(uint64_t, uint32_t, 128); }
+    else if (sw == 16) { if (nt == 256) PA_MSC(uint64_t, uint16_t, 256); else PA_MSC(uint64_t, uint16_t, 128); }
+    else { if (nt == 256) PA_MSC(uint64_t, uint8_t, 256); else PA_MSC(uint64_t, uint8_t, 128); }
+  }
+#undef PA_MSC
+  return hipGetLastError();
+}
+
+template <typename F>
+static hipError_t allow_lds(F* f) {
+  return hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(160 * 1024 - 2048));
+}
+
+}  // namespace pa
+
+extern "C" {
+
+using namespace pa;
+
+// Raise the dynamic-LDS limit of every kernel that keeps a lattice in LDS.
+hipError_t pa_prepare_kernels(void) {
+  hipError_t e = hipSuccess;
+#define PA_ALLOW(...) if (e == hipSuccess) e = allow_lds(__VA_ARGS__)
+  PA_ALLOW(k_init_ssc);
+  PA_ALLOW(k_sweep_ssc<128>);
+  PA_ALLOW(k_sweep_ssc<256>);
+#define PA_ALLOW_MSC(WT, ST)                          \
+  PA_ALLOW(k_sweep_msc<WT, ST, 128, false>);          \
+  PA_ALLOW(k_sweep_msc<WT, ST, 256, false>);          \
+  PA_ALLOW(k_sweep_msc<WT, ST, 128, true>);           \
+  PA_ALLOW(k_sweep_msc<WT, ST, 256, true>)
+  PA_ALLOW_MSC(uint32_t, uint32_t);
+  PA_ALLOW_MSC(uint32_t, uint16_t);
+  PA_ALLOW_MSC(uint32_t, uint8_t);
+  PA_ALLOW_MSC(uint64_t, uint64_t);
+  PA_ALLOW_MSC(uint64_t, uint32_t);
+  PA_ALLOW_MSC(uint64_t, uint16_t);
+  PA_ALLOW_MSC(uint64_t, uint8_t);
+#undef PA_ALLOW_MSC
+#undef PA_ALLOW
+  return e;
+}
+
+// Initial population of a group of runs sharing (coding, w, slice, threads).
+hipError_t pa_launch_init(int coding, int w, int sw, int nt, int nblocks, size_t lds,
+                          hipStream_t s, const RunDesc* runs, const int32_t* offs, int nruns,
+                          const uint64_t* is0) {
+  if (nblocks <= 0) return hipSuccess;
+  if (coding == kSSC) {
+    hipLaunchKernelGGL(k_init_ssc, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns, is0);
+    return hipGetLastError();
+  }
+  if (w == 32)
+    hipLaunchKernelGGL(k_init_msc<uint32_t>, dim3(nblocks), dim3(128), 0, s, runs, offs, nruns, is0);
+  else
+    hipLaunchKernelGGL(k_init_msc<uint64_t>, dim3(nblocks), dim3(128), 0, s, runs, offs, nruns, is0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_sweep_msc<true>(w, sw, nt, nblocks * (w / sw), lds, s, runs, offs, nruns);
+}
+
+hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nblocks, size_t lds,
+                          hipStream_t s, const RunDesc* runs, const int32_t* offs, int nruns) {
+  if (nblocks <= 0) return hipSuccess;
+  switch (kind) {
+    case 0: hipLaunchKernelGGL(k_qpart, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
+    case 1: hipLaunchKernelGGL(k_tau, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
+    case 2: hipLaunchKernelGGL(k_scan, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
+    case 3: hipLaunchKernelGGL(k_resample, dim3(nblocks), dim3(256), 0, s, runs, offs, nruns); break;
+    case 4:
+      if (coding == kSSC) {
+        if (nt == 256) hipLaunchKernelGGL(k_sweep_ssc<256>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+        else hipLaunchKernelGGL(k_sweep_ssc<128>, dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+      } else {
+        return launch_sweep_msc<false>(w, sw, nt, nblocks, lds, s, runs, offs, nruns);
+      }
+      break;
+    case 5: hipLaunchKernelGGL(k_avg, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
+    case 6: hipLaunchKernelGGL(k_hash, dim3(nblocks), dim3(256), 0, s, runs, offs, nruns); break;
+    case 7: hipLaunchKernelGGL(k_fin, dim3(nblocks), dim3(64), 0, s, runs, nruns); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // extern "C"
