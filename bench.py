@@ -109,8 +109,8 @@ def bench_plain(steps, warmup):
     return attempts, wall, sweep_ms, rows0, rows
 
 
-def bench_batch(nruns=64, seed=2026):
-    """configs[3]-style mix: independent runs of uneven L, R, EqSw, both codings, one batch."""
+def batch_mix(nruns=64, seed=2026, steps=0):
+    """independent runs of uneven L, R and EqSw, both codings, dB=0.005, beta 0->1."""
     import random
     rng = random.Random(seed)
     plist = []
@@ -119,9 +119,90 @@ def bench_batch(nruns=64, seed=2026):
         Rk = rng.randint(1000, 50000)
         eq = rng.choice([10, 50, 100])
         coding = rng.choice([pa.SSC, pa.MSC])
-        plist.append(pa.make_params(L=Lk, coding=coding, msbits=32, Rinit=Rk, eqsweeps=eq,
-                                    dB=DB, Bfin=1.0, seed=100 + k))
+        plist.append(pa.make_params(L=Lk, coding=coding, msbits=32, Rinit=Rk, eqsweeps=eq, dB=DB,
+                                    Bfin=1.0, seed=100 + k, max_steps=steps))
     return plist
+
+
+def bench_batch(args):
+    """Batched path (one launch sequence per step for all runs, hipGraph replay) against the
+    plain path launched one run at a time, on the same runs; every batched run is checked
+    bit-exact against its plain counterpart."""
+    plist = batch_mix(args.nruns, steps=args.batch_steps)
+    t0 = time.perf_counter()
+    b = pa.Batch(plist, use_graph=True)
+    b.run()
+    t_batch = time.perf_counter() - t0
+    brows = [b.results(i) for i in range(len(plist))]
+    b.close()
+    t0 = time.perf_counter()
+    prows = [pa.run_all(p)[0] for p in plist]
+    t_plain = time.perf_counter() - t0
+    exact = all(pa.format_rows(x, 17) == pa.format_rows(y, 17) and
+                [r.spin_hash for r in x] == [r.spin_hash for r in y] for x, y in zip(brows, prows))
+    steps = sum(len(r) - 1 for r in brows)
+    return {
+        "metric": "batched PA runs/s (configs[3] mix) vs plain one-at-a-time",
+        "value": round(len(plist) / t_batch, 4), "unit": "runs/s", "n_gpus": 1,
+        "higher_is_better": True, "plain_runs_per_s": round(len(plist) / t_plain, 4),
+        "speedup_vs_plain": round(t_plain / t_batch, 3), "bit_exact_vs_plain": exact,
+        "wall_s_batched": round(t_batch, 3), "wall_s_plain": round(t_plain, 3),
+        "beta_steps_total": steps,
+        "config": {"workload": f"{len(plist)} runs, L in {{16,24,32,48,64,96}}, R in [1000,50000], "
+                               f"EqSw in {{10,50,100}}, SSC/MSC32 mixed, dB=0.005, beta 0->1"
+                               + (f", truncated to {args.batch_steps} steps" if args.batch_steps else "")},
+    }
+
+
+def serve_requests(n, seed=5):
+    """Poisson arrivals (20/s), L in {16..256 step 16}, R log-uniform in [1e3,1e5],
+    dB in {0.001, 0.005}, Bfin uniform in [0.44, 1.0], EqSw=50, MSC32."""
+    import random
+    rng = random.Random(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += rng.expovariate(20.0)
+        L = 16 * rng.randint(1, 16)
+        R = int(round(10 ** rng.uniform(3, 5)))
+        out.append((t, pa.make_params(L=L, coding=pa.MSC, msbits=32, Rinit=R, eqsweeps=50,
+                                      dB=rng.choice([0.001, 0.005]), Bfin=rng.uniform(0.44, 1.0),
+                                      seed=1000 + k)))
+    return out
+
+
+def bench_serve(args):
+    reqs = serve_requests(args.nreq)
+    srv = pa.Server(hbm_budget_bytes=200 * 10**9, max_runs=256, steps_per_tick=args.tick_steps)
+    t0 = time.perf_counter()
+    pending = list(reqs)
+    tickets = []
+    while pending or not all(srv.done(t) for t, _ in tickets):
+        now = time.perf_counter() - t0
+        while pending and pending[0][0] <= now:
+            _, p = pending.pop(0)
+            tickets.append((srv.submit(p), p))
+        if tickets:
+            srv.tick()
+        else:
+            time.sleep(min(0.001, max(0.0, pending[0][0] - now)))
+    wall = time.perf_counter() - t0
+    lat = sorted(srv.latency(t)[0] for t, _ in tickets)
+    def pct(q):
+        return lat[min(len(lat) - 1, int(math.ceil(q * len(lat))) - 1)]
+    exact = True
+    if args.check:
+        for t, p in tickets[: args.check]:
+            exact &= pa.format_rows(srv.results(t), 17) == pa.format_rows(pa.run_all(p)[0], 17)
+    srv.close()
+    return {
+        "metric": "served PA runs: p50/p99 completion latency (configs[4] arrivals)",
+        "value": round(pct(0.5), 4), "unit": "s (p50 latency)", "p99_s": round(pct(0.99), 4),
+        "runs_per_s": round(len(lat) / wall, 4), "n_requests": len(lat), "n_gpus": 1,
+        "higher_is_better": False, "bit_exact_vs_plain_checked": args.check if exact else "FAILED",
+        "config": {"workload": "Poisson 20 req/s, L in {16..256 step 16}, R log-uniform [1e3,1e5], "
+                               "dB in {0.001,0.005}, Bfin U[0.44,1], EqSw=50, MSC32, 200 GB budget"},
+    }
 
 
 def main():
@@ -129,13 +210,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="plain", choices=["plain"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nruns", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=0, help="truncate batched runs (0 = full)")
+    ap.add_argument("--nreq", type=int, default=16)
+    ap.add_argument("--tick-steps", type=int, default=8)
+    ap.add_argument("--check", type=int, default=4, help="served runs re-run on the plain path")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs each PA population on a single "
               "device (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "batch":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if args.mode == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
+        return
     name, cu, _ = pa.device_info()
     clock = device_clock_hz()
     attempts, wall, sweep_ms, rows0, rows = bench_plain(args.steps, args.warmup)

@@ -56,7 +56,7 @@ typedef struct pa_params {
   int32_t max_steps;         /* 0 = whole schedule; otherwise stop after this many steps  */
   int32_t trace;             /* 1 = record per-step population hashes (parity checks)     */
   int32_t slice_bits;        /* MSC sweep slice width 8/16/32/64, 0 = autotuned           */
-  int32_t threads;           /* sweep workgroup size 128/256, 0 = autotuned               */
+  int32_t threads;           /* sweep workgroup size 128/256/512, 0 = autotuned           */
   int32_t reserved;
 } pa_params;
 

@@ -52,6 +52,8 @@ struct RunDesc {
   uint32_t eqsweeps;
   int32_t trace;            // compute per-step hashes
   int32_t q128, r128;       // 128 / Ld2, 128 % Ld2 (site-walk increments)
+  uint32_t ld2_magic;       // floor(2^32 / Ld2) + 1: idx / Ld2 == mulhi(idx, magic) for idx < 2^17
+  int32_t pad1;
   uint64_t seed;
   void* pop[2];             // spin buffers: SSC int8 [cap][N], MSC word [cap][N]
   uint64_t* amask[2];       // MSC active-lane bit masks per group (ping-pong with pop)
@@ -59,6 +61,10 @@ struct RunDesc {
   int32_t* M;               // magnetisations [cap_pop]
   uint32_t* Roff;           // copy counts [cap_pop]
   uint32_t* pre;            // exclusive prefix of Roff [cap_pop]
+  int32_t* par;             // parent of every child of the last resampling [cap_pop]
+  double* gsum;             // MSC per-group lane sums of the q terms [cap_units]
+  int32_t rs_sites;         // MSC resampling: site words per workgroup (64-byte chunks)
+  int32_t pad0;
   uint32_t* cblk;           // per-1024 chunk sums of Roff
   double* qblk;             // per-1024 chunk partial q
   double* avblk;            // per (lane, chunk) partial averages, 5 each
@@ -78,7 +84,8 @@ enum KernelKind : int32_t {
   kKChunks = 1,       // one block per 1024 population members (replicas / lanes)
   kKGroupChunks = 2,  // q partials: 1024 replicas (SSC) or 1024 groups (MSC)
   kKAvg = 3,          // averages: 1024 replicas (SSC) or (1024 groups, lane) (MSC)
-  kKindCount = 4
+  kKResample = 4,     // resampling: one block per child replica (SSC) / site chunk (MSC)
+  kKindCount = 5
 };
 
 }  // namespace pa

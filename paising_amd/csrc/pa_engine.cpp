@@ -90,6 +90,7 @@ struct RunHost {
   int w = 1, N = 0, n_steps = 0, cap_pop = 0, cap_units = 0;
   int R0 = 0;
   int sw = 0, nt = 0;
+  int rs_sites = 1;
   std::vector<double> B;
   size_t footprint = 0;
   // device memory (one allocation carved into pieces)
@@ -109,8 +110,12 @@ struct RunHost {
       case kKGroupChunks: return ceil_div(cap_units, kRedBlock);
       case kKAvg: return p.coding == kSSC ? ceil_div(cap_pop, kRedBlock)
                                           : ceil_div(cap_units, kRedBlock) * w;
+      case kKResample: return p.coding == kSSC ? cap_units : ceil_div(N, rs_sites);
     }
     return 0;
+  }
+  size_t resample_lds() const {
+    return p.coding == kSSC ? 0 : (size_t)cap_units * rs_sites * (w / 8);
   }
 };
 
@@ -143,8 +148,11 @@ int plan_run(const pa_params& p, RunHost& h, double cap_sigma) {
   // sweep variant
   const int Nh = h.N / 2;
   const bool aligned = (Nh % 512) == 0;
-  h.nt = p.threads ? p.threads : (aligned ? 256 : 128);
-  if (h.nt != 128 && h.nt != 256) return fail(PA_ERR_INVALID, "threads must be 128 or 256");
+  // aligned lattices: 128 streams x P parts, P = blocks per stream per half sweep, <= 4
+  const int nb = Nh / 512;
+  h.nt = p.threads ? p.threads : (aligned ? 128 * std::min(4, std::max(1, nb)) : 128);
+  if (h.nt != 128 && h.nt != 256 && h.nt != 512)
+    return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512");
   if (!aligned) h.nt = 128;
   if (p.coding == PA_MSC) {
     int sw = p.slice_bits;
@@ -156,6 +164,11 @@ int plan_run(const pa_params& p, RunHost& h, double cap_sigma) {
     if (sw > h.w) sw = h.w;
     if ((size_t)h.N * (sw / 8) + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice slice exceeds LDS");
     h.sw = sw;
+    // resampling stages a 64-byte chunk of every parent group in LDS (<= 96 KiB)
+    h.rs_sites = 64 / (h.w / 8);
+    while (h.rs_sites > 1 && (size_t)h.cap_units * h.rs_sites * (h.w / 8) > 96 * 1024) h.rs_sites /= 2;
+    if ((size_t)h.cap_units * h.rs_sites * (h.w / 8) > 150 * 1024)
+      return fail(PA_ERR_INVALID, "population too large for one device resampling chunk");
   } else {
     if ((size_t)h.N + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice exceeds LDS");
     h.sw = 8;
@@ -179,6 +192,8 @@ size_t layout(RunHost& h, uint8_t* base) {
   d.M = (int32_t*)take((size_t)h.cap_pop * 4);
   d.Roff = (uint32_t*)take((size_t)h.cap_pop * 4);
   d.pre = (uint32_t*)take((size_t)h.cap_pop * 4);
+  d.par = (int32_t*)take((size_t)h.cap_pop * 4);
+  d.gsum = (double*)take((size_t)h.cap_units * 8);
   d.cblk = (uint32_t*)take((size_t)h.blocks(kKChunks) * 4);
   d.qblk = (double*)take((size_t)h.blocks(kKGroupChunks) * 8 + 8 * (size_t)h.blocks(kKChunks));
   d.avblk = (double*)take((size_t)h.blocks(kKAvg) * 5 * 8);
@@ -202,6 +217,8 @@ int materialize(RunHost& h) {
   d.coding = h.p.coding; d.w = h.w; d.Rinit = h.p.Rinit; d.n_steps = h.n_steps;
   d.cap_units = h.cap_units; d.umap = h.p.uniform; d.eqsweeps = h.p.eqsweeps;
   d.trace = h.p.trace; d.q128 = 128 / d.Ld2; d.r128 = 128 % d.Ld2; d.seed = h.p.seed;
+  d.rs_sites = h.rs_sites;
+  d.ld2_magic = (uint32_t)((1ull << 32) / (uint64_t)d.Ld2 + 1);
   std::vector<double> dbeta(h.n_steps + 1, 0.0);
   std::vector<uint64_t> thr((size_t)(h.n_steps + 1) * 3, 0);
   for (int i = 1; i <= h.n_steps; ++i) {
@@ -254,7 +271,9 @@ struct Batch {
   RunDesc* desc_d = nullptr;
   int32_t* offs_d = nullptr;
   uint64_t* is0_d = nullptr;
-  int blocks[kKindCount] = {0, 0, 0, 0};
+  int blocks[kKindCount] = {0, 0, 0, 0, 0};
+  size_t rs_lds[2] = {0, 0};  // MSC resampling LDS for w = 32, 64
+  bool has_w[2] = {false, false};
   std::vector<Group> groups;
   bool any_trace = false;
   bool use_graph = false;
@@ -308,9 +327,16 @@ struct Batch {
       offs[(size_t)k * (n + 1) + n] = acc;
       blocks[k] = acc;
     }
+    rs_lds[0] = rs_lds[1] = 0;
+    has_w[0] = has_w[1] = false;
     for (int r = 0; r < n; ++r) {
       descs[r] = runs[r]->desc;
       is0[r] = runs[r]->p.initial_sequence;
+      if (runs[r]->p.coding == PA_MSC) {
+        const int wi = runs[r]->w == 32 ? 0 : 1;
+        has_w[wi] = true;
+        rs_lds[wi] = std::max(rs_lds[wi], runs[r]->resample_lds());
+      }
       any_trace |= runs[r]->p.trace != 0;
       max_steps = std::max(max_steps, runs[r]->n_steps);
     }
@@ -366,6 +392,12 @@ struct Batch {
   }
   std::vector<uint64_t*> init_is0;
 
+  bool has_ssc() const {
+    for (const RunHost* h : runs)
+      if (h->p.coding == PA_SSC) return true;
+    return false;
+  }
+
   int launch_init() {
     for (size_t i = 0; i < groups.size(); ++i) {
       const Group& g = groups[i];
@@ -378,10 +410,18 @@ struct Batch {
   // One beta step of every active run (8 launches + one sweep launch per variant).
   int launch_step(bool time_sweep) {
     const int n = (int)runs.size();
+    if (has_w[0] || has_w[1])
+      HIPCHECK(pa_launch_step(8, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(1, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(2, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
-    HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, blocks[kKUnits], 0, stream, desc_d, offs_d, n));
+    HIPCHECK(pa_launch_step(9, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
+    if (has_w[0] || has_w[1] ? has_ssc() : true)
+      HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, blocks[kKResample], 0, stream, desc_d, offs_d, n));
+    for (int wi = 0; wi < 2; ++wi)
+      if (has_w[wi])
+        HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, 0, blocks[kKResample], rs_lds[wi], stream,
+                                desc_d, offs_d, n));
     if (time_sweep) HIPCHECK(hipEventRecord(evs0, stream));
     for (const Group& g : groups) {
       const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;

@@ -91,6 +91,12 @@ __device__ __forceinline__ Site site_of(int idx, int side, int ty, int tx, int L
   return s;
 }
 
+__device__ __forceinline__ Site site_at(int idx, int side, const RunDesc& d) {
+  const int ty = (int)__umulhi((uint32_t)idx, d.ld2_magic);
+  const int tx = idx - ty * d.Ld2;
+  return site_of(idx, side, ty, tx, d.L, d.Ld2, d.Nh);
+}
+
 // ---- Boltzmann acceptance, SSC ----------------------------------------------------------
 // u < T  <=>  v < V(T) for the 32-bit word v behind u; V is computed on the host per step
 // for T = 1 (dE <= 0), exp(-4 beta), exp(-8 beta).
@@ -272,10 +278,36 @@ __global__ __launch_bounds__(128) void k_init_msc(const RunDesc* __restrict__ ru
 
 // ============================================================================================
 // Q partial sums: exp(-dB (E - Emean)) summed in the reference tree order.
-// SSC (QKer part 0, SSC:220-224): one block per 1024 replicas.
-// MSC (QKer parts 0+1, MSC:321-335): per group a w-thread block sum over lanes, then one
-// block per 1024 groups.  One thread per group here evaluates its group's lane tree.
+// MSC lane sums (QKer part 0, MSC:321-329): a block of w threads per group; here one lane
+// per thread, the warp-32 shuffle tree per 32 lanes, and for w = 64 the two warp sums
+// added as warp 0 of a 64-thread block does.
 // ============================================================================================
+__global__ __launch_bounds__(1024) void k_qgroup(const RunDesc* __restrict__ runs,
+                                                 const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKChunks * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (d.coding != kMSC || !st->active || st->overflow) return;
+  const int c = blockIdx.x - off[r];
+  const int R = st->R, w = d.w, rg = ceil_div(R, w);
+  if (c >= ceil_div(rg * w, kRB)) return;
+  const int n = c * kRB + threadIdx.x;
+  const int g = n / w, lane = n % w;
+  const double dB = d.dbeta[st->step], Emean = st->Emean;
+  double f = 0.0;
+  if (g < rg && ((d.amask[st->buf][g] >> lane) & 1))
+    f = exp_fp64(-dB * ((double)d.IE[n] - Emean));
+  f = warp32_tree(f);
+  if (w == 64) {
+    const double hi = __shfl_down(f, 32, 64);
+    f = f + hi;
+  }
+  if (lane == 0 && g < rg) d.gsum[g] = f;
+}
+
+// SSC (QKer part 0, SSC:220-224): one block per 1024 replicas.
+// MSC (QKer part 1, MSC:330-335): one block per 1024 group sums.
 __global__ __launch_bounds__(1024) void k_qpart(const RunDesc* __restrict__ runs,
                                                 const int32_t* __restrict__ offs, int nruns) {
   const int* off = offs + kKGroupChunks * (nruns + 1);
@@ -293,28 +325,10 @@ __global__ __launch_bounds__(1024) void k_qpart(const RunDesc* __restrict__ runs
     const int idx = c * kRB + threadIdx.x;
     f = (idx < R) ? exp_fp64(-dB * ((double)d.IE[idx] - Emean)) : 0.0;
   } else {
-    const int w = d.w, rg = ceil_div(R, w);
+    const int rg = ceil_div(R, d.w);
     if (c >= ceil_div(rg, kRB)) return;
     const int g = c * kRB + threadIdx.x;
-    if (g < rg) {
-      const uint64_t mask = d.amask[st->buf][g];
-      double hsum[2];
-      for (int h = 0; h < w / 32; ++h) {
-        double v[32];
-#pragma unroll
-        for (int i = 0; i < 32; ++i) {
-          const int br = 32 * h + i;
-          v[i] = ((mask >> br) & 1) ? exp_fp64(-dB * ((double)d.IE[(size_t)g * w + br] - Emean))
-                                    : 0.0;
-        }
-#pragma unroll
-        for (int o = 16; o > 0; o >>= 1)
-#pragma unroll
-          for (int i = 0; i < o; ++i) v[i] = v[i] + v[i + o];
-        hsum[h] = v[0];
-      }
-      f = (w == 32) ? hsum[0] : hsum[0] + hsum[1];
-    }
+    f = (g < rg) ? d.gsum[g] : 0.0;
   }
   const double s = ref_block_sum(f, sh);
   if (threadIdx.x == 0) {
@@ -443,95 +457,110 @@ __global__ __launch_bounds__(1024) void k_scan(const RunDesc* __restrict__ runs,
   }
 }
 
-// Parent of child j: the last parent p with pre[p] <= j (pre is non-decreasing, the true
-// parent satisfies pre[p] <= j < pre[p] + Roff[p]).
-__device__ __forceinline__ int parent_of(const uint32_t* __restrict__ pre, int n, uint32_t j) {
-  int lo = 0, hi = n;  // pre[lo] <= j always (pre[0] = 0)
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (pre[mid] <= j) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
-// ============================================================================================
-// k_resample.  SSC (resampleKer SSC:272-280): child j receives its parent's N spins
-// (children of a parent are contiguous and in parent order).  All N sites are copied.
-// MSC (MSC:397-435): child n = prefix + p lives in new group n mod rg_new, lane n div rg_new;
-// each new word gathers one bit from each lane's parent (same site).
-// One block per unit of the new population.
-// ============================================================================================
-__global__ __launch_bounds__(256) void k_resample(const RunDesc* __restrict__ runs,
+// Parent of every child: children of parent p are pre[p] .. pre[p] + Roff[p] - 1, in
+// parent order (CalcParSum + resampleKer, SSC:256-280 / MSC:362-435).
+__global__ __launch_bounds__(1024) void k_parents(const RunDesc* __restrict__ runs,
                                                   const int32_t* __restrict__ offs, int nruns) {
-  const int* off = offs + kKUnits * (nruns + 1);
+  const int* off = offs + kKChunks * (nruns + 1);
   const int r = find_run(off, nruns, blockIdx.x);
   const RunDesc& d = runs[r];
   RunState* st = d.st;
   if (!st->active || st->overflow) return;
+  const int c = blockIdx.x - off[r];
+  const int R = st->R, w = d.w;
+  const int nel = (d.coding == kSSC) ? R : ceil_div(R, w) * w;
+  const int p = c * kRB + threadIdx.x;
+  if (p >= nel) return;
+  const uint32_t k = d.Roff[p], b = d.pre[p];
+  for (uint32_t i = 0; i < k; ++i) d.par[b + i] = p;
+}
+
+// SSC (resampleKer SSC:272-280): child j receives its parent's N spins; all N sites are
+// copied (the reference grid N/2/128 drops N/2 mod 128 sites when L is not a multiple
+// of 16).  One block per child.
+__global__ __launch_bounds__(256) void k_resample(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKResample * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (d.coding != kSSC || !st->active || st->overflow) return;
   const int j = blockIdx.x - off[r];
-  const int R = st->R, Rn = st->Rnew, w = d.w, buf = st->buf;
-  if (d.coding == kSSC) {
-    if (j >= Rn) return;
-    __shared__ int par_sh;
-    if (threadIdx.x == 0) par_sh = parent_of(d.pre, R, (uint32_t)j);
-    __syncthreads();
-    const int p = par_sh;
-    const uint8_t* src = (const uint8_t*)d.pop[buf] + (size_t)p * d.N;
-    uint8_t* dst = (uint8_t*)d.pop[buf ^ 1] + (size_t)j * d.N;
-    if ((d.N & 15) == 0) {
-      const uint4* s4 = (const uint4*)src;
-      uint4* d4 = (uint4*)dst;
-      for (int i = threadIdx.x; i < d.N / 16; i += blockDim.x) d4[i] = s4[i];
-    } else {
-      for (int i = threadIdx.x; i < d.N; i += blockDim.x) dst[i] = src[i];
-    }
-    return;
-  }
-  const int rgn = ceil_div(Rn, w);
-  if (j >= rgn) return;
-  const int nold = ceil_div(R, w) * w;
-  __shared__ int pg[64], pb[64];
-  if (threadIdx.x < (unsigned)w) {
-    const int64_t n = (int64_t)threadIdx.x * rgn + j;
-    if (n < Rn) {
-      const int p = parent_of(d.pre, nold, (uint32_t)n);
-      pg[threadIdx.x] = p / w;
-      pb[threadIdx.x] = p % w;
-    } else {
-      pg[threadIdx.x] = -1;
-      pb[threadIdx.x] = 0;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t m = 0;
-    for (int k = 0; k < w; ++k)
-      if (pg[k] >= 0) m |= 1ull << k;
-    d.amask[buf ^ 1][j] = m;
-  }
-  __syncthreads();
-  const int N = d.N;
-  if (w == 32) {
-    const uint32_t* src = (const uint32_t*)d.pop[buf];
-    uint32_t* dst = (uint32_t*)d.pop[buf ^ 1] + (size_t)j * N;
-    for (int s = threadIdx.x; s < N; s += blockDim.x) {
-      uint32_t acc = 0;
-      for (int k = 0; k < 32; ++k) {
-        const int g = pg[k];
-        if (g >= 0) acc |= ((src[(size_t)g * N + s] >> pb[k]) & 1u) << k;
-      }
-      dst[s] = acc;
-    }
+  if (j >= st->Rnew) return;
+  const int p = d.par[j];
+  const int buf = st->buf;
+  const uint8_t* src = (const uint8_t*)d.pop[buf] + (size_t)p * d.N;
+  uint8_t* dst = (uint8_t*)d.pop[buf ^ 1] + (size_t)j * d.N;
+  if ((d.N & 15) == 0) {
+    const uint4* s4 = (const uint4*)src;
+    uint4* d4 = (uint4*)dst;
+    for (int i = threadIdx.x; i < d.N / 16; i += blockDim.x) d4[i] = s4[i];
   } else {
-    const uint64_t* src = (const uint64_t*)d.pop[buf];
-    uint64_t* dst = (uint64_t*)d.pop[buf ^ 1] + (size_t)j * N;
-    for (int s = threadIdx.x; s < N; s += blockDim.x) {
-      uint64_t acc = 0;
-      for (int k = 0; k < 64; ++k) {
-        const int g = pg[k];
-        if (g >= 0) acc |= ((src[(size_t)g * N + s] >> pb[k]) & 1ull) << k;
-      }
-      dst[s] = acc;
+    for (int i = threadIdx.x; i < d.N; i += blockDim.x) dst[i] = src[i];
+  }
+}
+
+// MSC (resampleKer MSC:397-435): child n = prefix + p lives in new group n mod rg_new, lane
+// n div rg_new.  One block per chunk of S site words (S * sizeof(W) = 64 bytes where the
+// parents' chunks fit in LDS): the chunk of every parent group is staged in LDS once, then
+// every new word of the chunk gathers its w bits from there.  Each byte of the parent
+// population is read once and each byte of the children written once.
+template <typename W>
+__global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict__ runs,
+                                                      const int32_t* __restrict__ offs, int nruns) {
+  constexpr int w = (int)sizeof(W) * 8;
+  const int* off = offs + kKResample * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  if (d.coding != kMSC || d.w != w) return;
+  const int c = blockIdx.x - off[r];
+  const int S = d.rs_sites, N = d.N;
+  const int s0 = c * S;
+  if (s0 >= N) return;
+  const int ns = min(S, N - s0);
+  const int R = st->R, Rn = st->Rnew, buf = st->buf;
+  const int rgo = ceil_div(R, w), rgn = ceil_div(Rn, w);
+  extern __shared__ __align__(16) uint8_t smem_rs[];
+  W* old = (W*)smem_rs;
+  const W* src = (const W*)d.pop[buf];
+  W* dst = (W*)d.pop[buf ^ 1];
+  for (int i = threadIdx.x; i < rgo * S; i += blockDim.x) {
+    const int g = i / S, s = i - g * S;
+    if (s < ns) old[i] = src[(size_t)g * N + s0 + s];
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < rgn; j += blockDim.x) {
+    // parents of the w lanes of new group j: independent loads issued together
+    int pl[w];
+#pragma unroll
+    for (int k = 0; k < w; ++k) {
+      const int64_t n = (int64_t)k * rgn + j;
+      pl[k] = (n < Rn) ? d.par[n] : -1;
+    }
+    W acc[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) acc[s] = 0;
+#pragma unroll
+    for (int k = 0; k < w; ++k) {
+      const int p = pl[k];
+      if (p < 0) continue;
+      const int pg = p / w, pb = p - pg * w;
+      const W* o = old + pg * S;
+#pragma unroll
+      for (int s = 0; s < 16; ++s)
+        if (s < ns) acc[s] |= ((o[s] >> pb) & (W)1) << k;
+    }
+    W* out = dst + (size_t)j * N + s0;
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      if (s < ns) out[s] = acc[s];
+    if (c == 0) {
+      uint64_t m = 0;
+      for (int k = 0; k < w; ++k)
+        if ((int64_t)k * rgn + j < Rn) m |= 1ull << k;
+      d.amask[buf ^ 1][j] = m;
     }
   }
 }
@@ -566,29 +595,61 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
   const uint64_t seed = d.seed;
   const uint32_t nsw = d.eqsweeps;
   __syncthreads();
+  // thresholds as 32-bit words: accept iff v < thr, or always for dE <= 0 when V(1) = 2^32
+  const uint32_t V1w = (uint32_t)min(V1, (uint64_t)0xFFFFFFFFu);
+  const uint32_t V4w = (uint32_t)V4, V8w = (uint32_t)V8;
+  const bool always1 = V1 > 0xFFFFFFFFull;
   if ((Nh & 511) == 0) {
     // aligned: thread = (stream t, part); blocks m = part, part + P, ... of nb per half
     constexpr int P = NT / kStreamsPerUnit;
     const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
     const int nb = Nh / 512;
-    for (uint32_t s = 0; s < nsw; ++s) {
-      for (int h = 0; h < 2; ++h) {
-        int8_t* own = lat8 + h * Nh;
-        const int8_t* oth = lat8 + (h ^ 1) * Nh;
-        for (int m = part; m < nb; m += P) {
-          const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
-          const uint32_t v[4] = {rb.x, rb.y, rb.z, rb.w};
+    auto update = [&](int8_t* own, const int8_t* oth, const Site& q, uint32_t v) {
+      const int sp = own[q.idx];
+      const int prod = sp * (oth[q.i1] + oth[q.idx] + oth[q.i3] + oth[q.i4]);
+      const uint32_t thr = prod == 4 ? V8w : (prod == 2 ? V4w : V1w);
+      const bool acc = (v < thr) || (always1 && prod <= 0);
+      own[q.idx] = (int8_t)(acc ? -sp : sp);
+    };
+    if (nb <= P) {
+      // at most one block per half sweep per thread: its four sites stay in registers
+      const bool act = part < nb;
+      Site qs[2][4];
 #pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-            const int idx = t + kStreamsPerUnit * (4 * m + jj);
-            const int ty = idx / Ld2, tx = idx - ty * Ld2;
-            const Site q = site_of(idx, h, ty, tx, L, Ld2, Nh);
-            const int sp = own[idx];
-            const int prod = sp * (oth[q.i1] + oth[idx] + oth[q.i3] + oth[q.i4]);
-            if (ssc_accept(prod, v[jj], V1, V4, V8)) own[idx] = (int8_t)-sp;
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+          qs[h][jj] = site_at(t + kStreamsPerUnit * (4 * part + jj), h, d);
+      for (uint32_t s = 0; s < nsw; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (act) {
+            int8_t* own = lat8 + h * Nh;
+            const int8_t* oth = lat8 + (h ^ 1) * Nh;
+            const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + part);
+            update(own, oth, qs[h][0], rb.x);
+            update(own, oth, qs[h][1], rb.y);
+            update(own, oth, qs[h][2], rb.z);
+            update(own, oth, qs[h][3], rb.w);
           }
+          __syncthreads();
         }
-        __syncthreads();
+      }
+    } else {
+      for (uint32_t s = 0; s < nsw; ++s) {
+        for (int h = 0; h < 2; ++h) {
+          int8_t* own = lat8 + h * Nh;
+          const int8_t* oth = lat8 + (h ^ 1) * Nh;
+          for (int m = part; m < nb; m += P) {
+            const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+            const int base = t + kStreamsPerUnit * 4 * m;
+            update(own, oth, site_at(base, h, d), rb.x);
+            update(own, oth, site_at(base + kStreamsPerUnit, h, d), rb.y);
+            update(own, oth, site_at(base + 2 * kStreamsPerUnit, h, d), rb.z);
+            update(own, oth, site_at(base + 3 * kStreamsPerUnit, h, d), rb.w);
+          }
+          __syncthreads();
+        }
       }
     }
   } else {
@@ -606,12 +667,12 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
           const uint64_t p = (uint64_t)(2 * s + h) * nt + k;
           if ((p >> 2) != cached) { cached = p >> 2; rb = stream_block(seed, sub0 + t, cached); }
           const uint32_t v = u4_word(rb, (int)(p & 3));
-          const int idx = t + kStreamsPerUnit * k;
-          const int ty = idx / Ld2, tx = idx - ty * Ld2;
-          const Site q = site_of(idx, h, ty, tx, L, Ld2, Nh);
-          const int sp = own[idx];
-          const int prod = sp * (oth[q.i1] + oth[idx] + oth[q.i3] + oth[q.i4]);
-          if (ssc_accept(prod, v, V1, V4, V8)) own[idx] = (int8_t)-sp;
+          const Site q = site_at(t + kStreamsPerUnit * k, h, d);
+          const int sp = own[q.idx];
+          const int prod = sp * (oth[q.i1] + oth[q.idx] + oth[q.i3] + oth[q.i4]);
+          const uint32_t thr = prod == 4 ? V8w : (prod == 2 ? V4w : V1w);
+          const bool acc = (v < thr) || (always1 && prod <= 0);
+          own[q.idx] = (int8_t)(acc ? -sp : sp);
         }
         __syncthreads();
       }
@@ -679,25 +740,48 @@ __global__ __launch_bounds__(NT) void k_sweep_msc(const RunDesc* __restrict__ ru
     constexpr int P = NT / kStreamsPerUnit;
     const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
     const int nb = Nh / 512;
-    for (uint32_t s = 0; s < nsw; ++s) {
-      for (int h = 0; h < 2; ++h) {
-        S* own = lat + h * Nh;
-        const S* oth = lat + (h ^ 1) * Nh;
-        for (int m = part; m < nb; m += P) {
-          const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
-          uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
-          S c4[4], c8[4];
-          lcg_masks<S, 4>(ran, T4, T8, c4, c8);
+    auto block4 = [&](S* own, const S* oth, const Site (&q)[4], const U4& rb) {
+      uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
+      S c4[4], c8[4];
+      lcg_masks<S, 4>(ran, T4, T8, c4, c8);
 #pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-            const int idx = t + kStreamsPerUnit * (4 * m + jj);
-            const int ty = idx / Ld2, tx = idx - ty * Ld2;
-            const Site q = site_of(idx, h, ty, tx, L, Ld2, Nh);
-            const S sp = own[idx];
-            own[idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[idx], oth[q.i3], oth[q.i4], c4[jj], c8[jj]);
-          }
+      for (int jj = 0; jj < 4; ++jj) {
+        const S sp = own[q[jj].idx];
+        own[q[jj].idx] = sp ^ msc_acc<S>(sp, oth[q[jj].i1], oth[q[jj].idx], oth[q[jj].i3],
+                                         oth[q[jj].i4], c4[jj], c8[jj]);
+      }
+    };
+    if (nb <= P) {
+      // at most one block per half sweep per thread: its four sites stay in registers
+      const bool act = part < nb;
+      Site qs[2][4];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+          qs[h][jj] = site_at(t + kStreamsPerUnit * (4 * part + jj), h, d);
+      for (uint32_t s = 0; s < nsw; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (act)
+            block4(lat + h * Nh, lat + (h ^ 1) * Nh, qs[h],
+                   stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + part));
+          __syncthreads();
         }
-        __syncthreads();
+      }
+    } else {
+      for (uint32_t s = 0; s < nsw; ++s) {
+        for (int h = 0; h < 2; ++h) {
+          for (int m = part; m < nb; m += P) {
+            const int base = t + kStreamsPerUnit * 4 * m;
+            const Site q[4] = {site_at(base, h, d), site_at(base + kStreamsPerUnit, h, d),
+                               site_at(base + 2 * kStreamsPerUnit, h, d),
+                               site_at(base + 3 * kStreamsPerUnit, h, d)};
+            block4(lat + h * Nh, lat + (h ^ 1) * Nh, q,
+                   stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m));
+          }
+          __syncthreads();
+        }
       }
     }
   } else {
@@ -716,11 +800,9 @@ __global__ __launch_bounds__(NT) void k_sweep_msc(const RunDesc* __restrict__ ru
           uint32_t ran[1] = {JA * u4_word(rb, (int)(p & 3)) + JC};
           S c4[1], c8[1];
           lcg_masks<S, 1>(ran, T4, T8, c4, c8);
-          const int idx = t + kStreamsPerUnit * k;
-          const int ty = idx / Ld2, tx = idx - ty * Ld2;
-          const Site q = site_of(idx, h, ty, tx, L, Ld2, Nh);
-          const S sp = own[idx];
-          own[idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[idx], oth[q.i3], oth[q.i4], c4[0], c8[0]);
+          const Site q = site_at(t + kStreamsPerUnit * k, h, d);
+          const S sp = own[q.idx];
+          own[q.idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[q.idx], oth[q.i3], oth[q.i4], c4[0], c8[0]);
         }
         __syncthreads();
       }
@@ -864,16 +946,23 @@ static hipError_t launch_sweep_msc(int w, int sw, int nt, int nblocks, size_t ld
 #define PA_MSC(WT, ST, NTV)                                                                    \
   hipLaunchKernelGGL((k_sweep_msc<WT, ST, NTV, INIT>), dim3(nblocks), dim3(NTV), lds, s, runs, \
                      offs, nruns)
+#define PA_MSC_NT(WT, ST)                   \
+  do {                                      \
+    if (nt == 512) PA_MSC(WT, ST, 512);     \
+    else if (nt == 256) PA_MSC(WT, ST, 256); \
+    else PA_MSC(WT, ST, 128);               \
+  } while (0)
   if (w == 32) {
-    if (sw == 32) { if (nt == 256) PA_MSC(uint32_t, uint32_t, 256); else PA_MSC(uint32_t, uint32_t, 128); }
-    else if (sw == 16) { if (nt == 256) PA_MSC(uint32_t, uint16_t, 256); else PA_MSC(uint32_t, uint16_t, 128); }
-    else { if (nt == 256) PA_MSC(uint32_t, uint8_t, 256); else PA_MSC(uint32_t, uint8_t, 128); }
+    if (sw == 32) PA_MSC_NT(uint32_t, uint32_t);
+    else if (sw == 16) PA_MSC_NT(uint32_t, uint16_t);
+    else PA_MSC_NT(uint32_t, uint8_t);
   } else {
-    if (sw == 64) { if (nt == 256) PA_MSC(uint64_t, uint64_t, 256); else PA_MSC(uint64_t, uint64_t, 128); }
-    else if (sw == 32) { if (nt == 256) PA_MSC(uint64_t, uint32_t, 256); else PA_MSC(uint64_t, uint32_t, 128); }
-    else if (sw == 16) { if (nt == 256) PA_MSC(uint64_t, uint16_t, 256); else PA_MSC(uint64_t, uint16_t, 128); }
-    else { if (nt == 256) PA_MSC(uint64_t, uint8_t, 256); else PA_MSC(uint64_t, uint8_t, 128); }
+    if (sw == 64) PA_MSC_NT(uint64_t, uint64_t);
+    else if (sw == 32) PA_MSC_NT(uint64_t, uint32_t);
+    else if (sw == 16) PA_MSC_NT(uint64_t, uint16_t);
+    else PA_MSC_NT(uint64_t, uint8_t);
   }
+#undef PA_MSC_NT
 #undef PA_MSC
   return hipGetLastError();
 }
@@ -895,13 +984,18 @@ hipError_t pa_prepare_kernels(void) {
   hipError_t e = hipSuccess;
 #define PA_ALLOW(...) if (e == hipSuccess) e = allow_lds(__VA_ARGS__)
   PA_ALLOW(k_init_ssc);
+  PA_ALLOW(k_resample_msc<uint32_t>);
+  PA_ALLOW(k_resample_msc<uint64_t>);
   PA_ALLOW(k_sweep_ssc<128>);
   PA_ALLOW(k_sweep_ssc<256>);
+  PA_ALLOW(k_sweep_ssc<512>);
 #define PA_ALLOW_MSC(WT, ST)                          \
   PA_ALLOW(k_sweep_msc<WT, ST, 128, false>);          \
   PA_ALLOW(k_sweep_msc<WT, ST, 256, false>);          \
+  PA_ALLOW(k_sweep_msc<WT, ST, 512, false>);          \
   PA_ALLOW(k_sweep_msc<WT, ST, 128, true>);           \
-  PA_ALLOW(k_sweep_msc<WT, ST, 256, true>)
+  PA_ALLOW(k_sweep_msc<WT, ST, 256, true>);           \
+  PA_ALLOW(k_sweep_msc<WT, ST, 512, true>)
   PA_ALLOW_MSC(uint32_t, uint32_t);
   PA_ALLOW_MSC(uint32_t, uint16_t);
   PA_ALLOW_MSC(uint32_t, uint8_t);
@@ -942,7 +1036,8 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 3: hipLaunchKernelGGL(k_resample, dim3(nblocks), dim3(256), 0, s, runs, offs, nruns); break;
     case 4:
       if (coding == kSSC) {
-        if (nt == 256) hipLaunchKernelGGL(k_sweep_ssc<256>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+        if (nt == 512) hipLaunchKernelGGL(k_sweep_ssc<512>, dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
+        else if (nt == 256) hipLaunchKernelGGL(k_sweep_ssc<256>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
         else hipLaunchKernelGGL(k_sweep_ssc<128>, dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
       } else {
         return launch_sweep_msc<false>(w, sw, nt, nblocks, lds, s, runs, offs, nruns);
@@ -951,6 +1046,12 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 5: hipLaunchKernelGGL(k_avg, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 6: hipLaunchKernelGGL(k_hash, dim3(nblocks), dim3(256), 0, s, runs, offs, nruns); break;
     case 7: hipLaunchKernelGGL(k_fin, dim3(nblocks), dim3(64), 0, s, runs, nruns); break;
+    case 8: hipLaunchKernelGGL(k_qgroup, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
+    case 9: hipLaunchKernelGGL(k_parents, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
+    case 10:
+      if (w == 32) hipLaunchKernelGGL(k_resample_msc<uint32_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+      else hipLaunchKernelGGL(k_resample_msc<uint64_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
