@@ -158,7 +158,7 @@ int plan_run(const pa_params& p, RunHost& h, double cap_sigma) {
     int sw = p.slice_bits;
     if (sw == 0) {
       sw = h.w;
-      while (sw > 8 && (size_t)h.N * (sw / 8) > 96 * 1024) sw /= 2;
+      while (sw > 8 && (size_t)h.N * (sw / 8) > 128 * 1024) sw /= 2;
     }
     if (sw != 8 && sw != 16 && sw != 32 && sw != 64) return fail(PA_ERR_INVALID, "slice_bits");
     if (sw > h.w) sw = h.w;
