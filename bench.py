@@ -109,6 +109,27 @@ def bench_plain(steps, warmup):
     return attempts, wall, sweep_ms, rows0, rows
 
 
+def bench_config3(args):
+    """MSC 64 replicas/uint64, L=256, R=200000, EqSw=200, dB=0.002, timed on
+    the first --steps beta steps after --warmup (per-attempt work does not depend on beta)."""
+    p = pa.make_params(L=256, coding=pa.MSC, msbits=64, Rinit=200000, eqsweeps=200, dB=0.002,
+                       Bfin=1.0, seed=100, max_steps=args.warmup + args.steps)
+    run = pa.Run(p)
+    run.step(args.warmup)
+    run.sync()
+    t0 = time.perf_counter()
+    run.step(args.steps)
+    run.sync()
+    wall = time.perf_counter() - t0
+    rows = run.results()
+    run.close()
+    att = 200 * 65536 * sum(r.R for r in rows[args.warmup + 1:])
+    return {"metric": "spin-flip attempts/ns, L=256 R=2e5 MSC64 (configs[2], truncated schedule)",
+            "value": round(att / (wall * 1e9), 2), "unit": "spin-flip attempts/ns",
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 2),
+            "R": [r.R for r in rows]}
+
+
 def batch_mix(nruns=64, seed=2026, steps=0):
     """independent runs of uneven L, R and EqSw, both codings, dB=0.005, beta 0->1."""
     import random
@@ -210,7 +231,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "copy", "config3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nruns", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=0, help="truncate batched runs (0 = full)")
@@ -227,6 +248,14 @@ def main():
         return
     if args.mode == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
+        return
+    if args.mode == "copy":
+        gbps = pa.bandwidth_probe(1 << 30, 10)
+        print(json.dumps({"metric": "streaming device copy (read + write)", "value": round(gbps, 1),
+                          "unit": "GB/s", "bytes": 1 << 30, "iters": 10}), flush=True)
+        return
+    if args.mode == "config3":
+        print(json.dumps(bench_config3(args)), flush=True)
         return
     name, cu, _ = pa.device_info()
     clock = device_clock_hz()

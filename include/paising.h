@@ -80,6 +80,11 @@ const char* pa_last_error(void);
 int pa_version(void);
 int pa_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes);
 
+/* Streaming device-to-device copy of `bytes` (read + write), `iters` times: achieved GB/s
+ * (counts read and written bytes).  No reference counterpart: the HBM reference point for
+ * the resampling kernels and for scaling the FETCH_SIZE / WRITE_SIZE counters. */
+int pa_bandwidth_probe(size_t bytes, int iters, double* gbps);
+
 /* Number of beta steps of the reference schedule B_i = B_{i-1} + dB while B_i <= Bfin
  * (PAisingSSC.cu:508-510,661). */
 int pa_schedule_steps(double Binit, double dB, double Bfin);

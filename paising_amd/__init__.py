@@ -55,6 +55,7 @@ SIGNATURES = {
     "pa_version": (_I, []),
     "pa_device_info": (_I, [ctypes.c_char_p, _SZ, _P, _P]),
     "pa_schedule_steps": (_I, [_D, _D, _D]),
+    "pa_bandwidth_probe": (_I, [_SZ, _I, _P]),
     "pa_run_create": (_I, [_P, _PP]),
     "pa_run_step": (_I, [_P, _I]),
     "pa_run_sync": (_I, [_P]),
@@ -133,6 +134,13 @@ def device_info():
     hbm = ctypes.c_size_t(0)
     _check(lib().pa_device_info(name, 256, ctypes.byref(cu), ctypes.byref(hbm)))
     return name.value.decode(), cu.value, hbm.value
+
+
+def bandwidth_probe(nbytes=1 << 30, iters=10):
+    """Achieved GB/s (read + write) of a streaming device copy."""
+    g = ctypes.c_double(0)
+    _check(lib().pa_bandwidth_probe(nbytes, iters, ctypes.byref(g)))
+    return g.value
 
 
 def _rows(buf, n):

@@ -30,6 +30,7 @@ extern "C" hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt
                                      size_t lds, hipStream_t s, const pa::RunDesc* runs,
                                      const int32_t* offs, int nruns);
 extern "C" hipError_t pa_prepare_kernels(void);
+extern "C" hipError_t pa_launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 namespace {
 
@@ -572,6 +573,31 @@ int pa_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes
   if (name && name_len) snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
   if (cu_count) *cu_count = prop.multiProcessorCount;
   if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+  return PA_OK;
+}
+
+int pa_bandwidth_probe(size_t bytes, int iters, double* gbps) {
+  if (bytes < 16 || iters < 1 || !gbps) return fail(PA_ERR_INVALID, "bad probe arguments");
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  HIPCHECK(hipMalloc(&a, bytes));
+  HIPCHECK(hipMalloc(&b, bytes));
+  HIPCHECK(hipMemset(a, 1, bytes));
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  HIPCHECK(pa_launch_copy(a, b, bytes, nullptr));
+  HIPCHECK(hipEventRecord(e0, nullptr));
+  for (int i = 0; i < iters; ++i) HIPCHECK(pa_launch_copy(a, b, bytes, nullptr));
+  HIPCHECK(hipEventRecord(e1, nullptr));
+  HIPCHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2.0 * bytes * iters / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return PA_OK;
 }
 

@@ -973,11 +973,26 @@ static hipError_t allow_lds(F* f) {
                              (int)(160 * 1024 - 2048));
 }
 
+// Streaming copy (16-byte vectors, grid-stride): reference point for HBM bandwidth and for
+// scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                              size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace pa
 
 extern "C" {
 
 using namespace pa;
+
+hipError_t pa_launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(256 * 64), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+  return hipGetLastError();
+}
 
 // Raise the dynamic-LDS limit of every kernel that keeps a lattice in LDS.
 hipError_t pa_prepare_kernels(void) {
