@@ -57,7 +57,7 @@ typedef struct pa_params {
   int32_t trace;             /* 1 = record per-step population hashes (parity checks)     */
   int32_t slice_bits;        /* MSC sweep slice width 8/16/32/64, 0 = autotuned           */
   int32_t threads;           /* sweep workgroup size 128/256/512, 0 = autotuned           */
-  int32_t reserved;
+  int32_t balance;           /* MSC makespan-balanced sweeps: 0 auto, 1 on, -1 off        */
 } pa_params;
 
 /* One row of PA_results_run_<r>.dat (PAisingSSC.cu:679-690) and the step internals. */
@@ -113,6 +113,14 @@ int pa_run_set_timing(pa_run* r, int on);
 /* Replay steps from a captured hipGraph (1 = on). */
 int pa_run_set_graph(pa_run* r, int on);
 int pa_run_destroy(pa_run* r);
+
+/* Autotuner (no reference counterpart: the reference fixes 128 threads per replica,
+ * PAisingSSC.cu:62).  Times every valid sweep variant (MSC slice width, workgroup size,
+ * makespan balancing) on a two-step trial of *p, writes the fastest into *tuned and caches it
+ * for this process: later runs of the same shape (coding, w, L, log2 Rinit) that leave
+ * slice_bits/threads/balance at 0 use it.  Results never depend on the variant. */
+int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms);
+int pa_autotune_clear(void);
 
 /* Whole run in one call: create, all steps, results (a reference run, PAisingSSC.cu:481-692). */
 int pa_run_all(const pa_params* p, pa_row* rows, int max_rows, int* nrows, uint64_t* final_is);

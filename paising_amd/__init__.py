@@ -31,7 +31,7 @@ class Params(ctypes.Structure):
         ("Binit", ctypes.c_double), ("Bfin", ctypes.c_double), ("seed", ctypes.c_uint64),
         ("initial_sequence", ctypes.c_uint64), ("uniform", ctypes.c_int32),
         ("max_steps", ctypes.c_int32), ("trace", ctypes.c_int32), ("slice_bits", ctypes.c_int32),
-        ("threads", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("threads", ctypes.c_int32), ("balance", ctypes.c_int32),
     ]
 
 
@@ -84,6 +84,8 @@ SIGNATURES = {
     "pa_server_latency": (_I, [_P, ctypes.c_int64, _P, _P]),
     "pa_server_destroy": (_I, [_P]),
     "pa_run_footprint": (_SZ, [_P]),
+    "pa_autotune": (_I, [_P, _P, _P]),
+    "pa_autotune_clear": (_I, []),
 }
 
 _lib = None
@@ -115,9 +117,18 @@ def _ptr(a):
 
 def make_params(L=64, coding=MSC, msbits=32, Rinit=20000, eqsweeps=100, dB=0.005, Binit=0.0,
                 Bfin=1.0, seed=100, initial_sequence=0, uniform=UNIFORM_CUDA101, max_steps=0,
-                trace=0, slice_bits=0, threads=0):
+                trace=0, slice_bits=0, threads=0, balance=0):
     return Params(L, coding, msbits, Rinit, eqsweeps, dB, Binit, Bfin, seed & (2**64 - 1),
-                  initial_sequence, uniform, max_steps, trace, slice_bits, threads, 0)
+                  initial_sequence, uniform, max_steps, trace, slice_bits, threads, balance)
+
+
+def autotune(params):
+    """Time the sweep variants for this run shape; returns (tuned params, best sweep ms).
+    The choice is cached in the library for later runs of the same shape."""
+    out = Params()
+    ms = ctypes.c_double(0)
+    _check(lib().pa_autotune(ctypes.byref(params), ctypes.byref(out), ctypes.byref(ms)))
+    return out, ms.value
 
 
 def schedule_steps(Binit, dB, Bfin):

@@ -27,6 +27,8 @@ struct RunState {
   double lnQ;        // lnQ_i of the step in progress
   int32_t overflow;  // set when R_i exceeds the buffer capacity; the run then stops
   int32_t active;    // 0 once the run has finished its schedule
+  int32_t err;       // device-side error bits (1: balanced-sweep hand-over wait timed out)
+  int32_t pad;
 };
 
 // Per-step record written by the finalize kernel (row i of the results file + internals).
@@ -62,9 +64,10 @@ struct RunDesc {
   uint32_t* Roff;           // copy counts [cap_pop]
   uint32_t* pre;            // exclusive prefix of Roff [cap_pop]
   int32_t* par;             // parent of every child of the last resampling [cap_pop]
+  uint32_t* flags;          // balanced sweeps: hand-over flag per (group, slice) [cap_units*8]
   double* gsum;             // MSC per-group lane sums of the q terms [cap_units]
   int32_t rs_sites;         // MSC resampling: site words per workgroup (64-byte chunks)
-  int32_t pad0;
+  int32_t rs_split;         // MSC resampling: workgroups sharing one site chunk (new groups split)
   uint32_t* cblk;           // per-1024 chunk sums of Roff
   double* qblk;             // per-1024 chunk partial q
   double* avblk;            // per (lane, chunk) partial averages, 5 each

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -30,6 +31,8 @@ extern "C" hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt
                                      size_t lds, hipStream_t s, const pa::RunDesc* runs,
                                      const int32_t* offs, int nruns);
 extern "C" hipError_t pa_prepare_kernels(void);
+extern "C" hipError_t pa_occupancy(int kind, int coding, int w, int sw, int nt, size_t lds,
+                                   int* blocks_per_cu);
 extern "C" hipError_t pa_launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 namespace {
@@ -92,6 +95,7 @@ struct RunHost {
   int R0 = 0;
   int sw = 0, nt = 0;
   int rs_sites = 1;
+  int rs_split = 1;
   std::vector<double> B;
   size_t footprint = 0;
   // device memory (one allocation carved into pieces)
@@ -111,7 +115,7 @@ struct RunHost {
       case kKGroupChunks: return ceil_div(cap_units, kRedBlock);
       case kKAvg: return p.coding == kSSC ? ceil_div(cap_pop, kRedBlock)
                                           : ceil_div(cap_units, kRedBlock) * w;
-      case kKResample: return p.coding == kSSC ? cap_units : ceil_div(N, rs_sites);
+      case kKResample: return p.coding == kSSC ? cap_units : ceil_div(N, rs_sites) * rs_split;
     }
     return 0;
   }
@@ -120,8 +124,38 @@ struct RunHost {
   }
 };
 
+// Autotuner cache: shape -> best (slice_bits, threads, balance) measured by pa_autotune.
+struct TuneKey {
+  int coding, w, L, rbucket;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(coding, w, L, rbucket) < std::tie(o.coding, o.w, o.L, o.rbucket);
+  }
+};
+struct TuneVal {
+  int slice_bits, threads, balance;
+  double ms;
+};
+std::map<TuneKey, TuneVal>& tune_cache() {
+  static std::map<TuneKey, TuneVal> c;
+  return c;
+}
+TuneKey tune_key(const pa_params& p) {
+  int rb = 0;
+  while ((1 << (rb + 1)) <= p.Rinit) ++rb;  // floor(log2 Rinit)
+  return TuneKey{p.coding, p.coding == PA_MSC ? p.msbits : 1, p.L, rb};
+}
+
 // Validate, size and compute the per-step tables of a run (no device work).
-int plan_run(const pa_params& p, RunHost& h, double cap_sigma) {
+int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma) {
+  pa_params p = p_in;
+  {
+    auto it = tune_cache().find(tune_key(p));
+    if (it != tune_cache().end()) {
+      if (!p.slice_bits) p.slice_bits = it->second.slice_bits;
+      if (!p.threads) p.threads = it->second.threads;
+      if (!p.balance) p.balance = it->second.balance;
+    }
+  }
   if (p.L < 4 || (p.L & 1) || p.L > 1024) return fail(PA_ERR_INVALID, "L must be even, 4..1024");
   if (p.coding != PA_SSC && p.coding != PA_MSC) return fail(PA_ERR_INVALID, "coding");
   if (p.coding == PA_MSC && p.msbits != 32 && p.msbits != 64)
@@ -170,6 +204,8 @@ int plan_run(const pa_params& p, RunHost& h, double cap_sigma) {
     while (h.rs_sites > 1 && (size_t)h.cap_units * h.rs_sites * (h.w / 8) > 96 * 1024) h.rs_sites /= 2;
     if ((size_t)h.cap_units * h.rs_sites * (h.w / 8) > 150 * 1024)
       return fail(PA_ERR_INVALID, "population too large for one device resampling chunk");
+    // about one new group per thread: the site chunk is re-read from L2 by each split
+    h.rs_split = std::max(1, std::min(16, ceil_div(h.cap_units, 256)));
   } else {
     if ((size_t)h.N + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice exceeds LDS");
     h.sw = 8;
@@ -194,6 +230,7 @@ size_t layout(RunHost& h, uint8_t* base) {
   d.Roff = (uint32_t*)take((size_t)h.cap_pop * 4);
   d.pre = (uint32_t*)take((size_t)h.cap_pop * 4);
   d.par = (int32_t*)take((size_t)h.cap_pop * 4);
+  d.flags = (uint32_t*)take((size_t)h.cap_units * 8 * 4);
   d.gsum = (double*)take((size_t)h.cap_units * 8);
   d.cblk = (uint32_t*)take((size_t)h.blocks(kKChunks) * 4);
   d.qblk = (double*)take((size_t)h.blocks(kKGroupChunks) * 8 + 8 * (size_t)h.blocks(kKChunks));
@@ -219,6 +256,7 @@ int materialize(RunHost& h) {
   d.cap_units = h.cap_units; d.umap = h.p.uniform; d.eqsweeps = h.p.eqsweeps;
   d.trace = h.p.trace; d.q128 = 128 / d.Ld2; d.r128 = 128 % d.Ld2; d.seed = h.p.seed;
   d.rs_sites = h.rs_sites;
+  d.rs_split = h.rs_split;
   d.ld2_magic = (uint32_t)((1ull << 32) / (uint64_t)d.Ld2 + 1);
   std::vector<double> dbeta(h.n_steps + 1, 0.0);
   std::vector<uint64_t> thr((size_t)(h.n_steps + 1) * 3, 0);
@@ -240,6 +278,7 @@ int materialize(RunHost& h) {
   HIPCHECK(hipMemcpy((void*)d.dbeta, dbeta.data(), (h.n_steps + 1) * 8, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy((void*)d.thr, thr.data(), thr.size() * 8, hipMemcpyHostToDevice));
   HIPCHECK(hipMemset(d.out, 0, (h.n_steps + 1) * sizeof(StepOut)));
+  HIPCHECK(hipMemset(d.flags, 0, (size_t)h.cap_units * 8 * 4));
   RunState st{};
   st.step = 1;
   st.R = h.p.Rinit;
@@ -250,6 +289,7 @@ int materialize(RunHost& h) {
   st.Emean = 0.0;                                                     // SSC:483
   st.lnQ = 0.0;
   st.overflow = 0;
+  st.err = 0;
   st.active = h.n_steps > 0 ? 1 : 0;
   HIPCHECK(hipMemcpy(d.st, &st, sizeof(RunState), hipMemcpyHostToDevice));
   h.st_h = st;
@@ -263,7 +303,52 @@ struct Group {  // runs sharing one sweep kernel variant
   int32_t* offs_d = nullptr;
   int units_blocks = 0;  // total kKUnits blocks
   size_t lds = 0;
+  int bal_workers = 0;   // > 0: makespan-balanced MSC sweep with this many workers
+  size_t bal_lds = 0;
 };
+
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
+// Balanced sweeps pay off when a launch holds only a few units per CU (whole units per CU
+// would leave CUs with n and n+1 units).  k workers per CU are made co-resident by padding
+// their LDS so that no more than k fit a CU; P = k * CUs stays below the expected unit
+// count so that a share always exceeds one unit (a unit is cut at most once).
+void plan_balance(Group& g, const std::vector<RunHost*>& runs) {
+  g.bal_workers = 0;
+  if (g.v.coding != PA_MSC) return;
+  int64_t units = 0;
+  int force = 0;
+  for (int m : g.members) {
+    const RunHost& h = *runs[m];
+    if (h.p.eqsweeps == 0) return;
+    units += (int64_t)h.cap_units * (h.w / h.sw);
+    force = h.p.balance ? h.p.balance : force;
+  }
+  if (force < 0) return;
+  const int cus = device_cus();
+  const int64_t expect = units * 9 / 10;
+  if (force == 0 && (expect < cus || expect >= 12 * (int64_t)cus)) return;
+  int k = (int)std::max<int64_t>(1, std::min<int64_t>(expect / cus, 4));
+  for (; k >= 1; --k) {
+    const size_t lds = std::max(g.lds, (size_t)(160 * 1024) / k - 4096);
+    int occ = 0;
+    if (pa_occupancy(11, g.v.coding, g.v.w, g.v.sw, g.v.nt, lds, &occ) != hipSuccess) return;
+    if (occ == k) { g.bal_lds = lds; break; }
+  }
+  if (k < 1) return;
+  g.bal_workers = k * cus;
+}
 
 struct Batch {
   std::vector<RunHost*> runs;  // not owned when borrowed by the server
@@ -385,6 +470,7 @@ struct Batch {
         uint64_t* gis_d = nullptr;
         if ((rc = alloc_table((void**)&gis_d, m * 8))) return rc;
         HIPCHECK(hipMemcpy(gis_d, gis.data(), m * 8, hipMemcpyHostToDevice));
+        plan_balance(g, runs);
         groups.push_back(g);
         init_is0.push_back(gis_d);
       }
@@ -426,8 +512,12 @@ struct Batch {
     if (time_sweep) HIPCHECK(hipEventRecord(evs0, stream));
     for (const Group& g : groups) {
       const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;
-      HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.units_blocks * ns, g.lds, stream,
-                              g.desc_d, g.offs_d, (int)g.members.size()));
+      if (g.bal_workers > 0)
+        HIPCHECK(pa_launch_step(11, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, stream,
+                                g.desc_d, g.offs_d, (int)g.members.size()));
+      else
+        HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.units_blocks * ns, g.lds, stream,
+                                g.desc_d, g.offs_d, (int)g.members.size()));
     }
     if (time_sweep) HIPCHECK(hipEventRecord(evs1, stream));
     HIPCHECK(pa_launch_step(5, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n));
@@ -493,6 +583,9 @@ int fetch(RunHost& h, hipStream_t s) {
                           hipMemcpyDeviceToHost, s));
   HIPCHECK(hipMemcpyAsync(&h.st_h, h.desc.st, sizeof(RunState), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
+  if (h.st_h.err)
+    return fail(PA_ERR_HIP, "balanced sweep hand-over wait timed out (device error bits " +
+                                std::to_string(h.st_h.err) + ")");
   if (h.st_h.overflow)
     return fail(PA_ERR_CAPACITY, "population outgrew its buffers");
   h.results_valid = true;
@@ -763,6 +856,71 @@ int pa_run_all(const pa_params* p, pa_row* rows, int max_rows, int* nrows, uint6
     return rc;
   }
   return fail(PA_ERR_CAPACITY, "population outgrew its buffers repeatedly");
+}
+
+// ---------------- autotuner ----------------
+// Times each candidate sweep variant (slice width, workgroup size, balanced or not) on a
+// short trial of the real run (2 steps, the second timed with HIP events around the sweep
+// launches) and caches the fastest for runs of the same shape.  All variants produce the
+// same bits, so tuning only changes speed.
+int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
+  if (!p || !tuned) return fail(PA_ERR_INVALID, "null argument");
+  RunHost probe;
+  int rc = plan_run(*p, probe, 10.0);
+  if (rc) return rc;
+  std::vector<std::array<int, 3>> cands;
+  const int Nh = p->L * p->L / 2;
+  const bool aligned = (Nh % 512) == 0;
+  const int nts[3] = {128, 256, 512};
+  for (int nt : nts) {
+    if (!aligned && nt != 128) continue;
+    if (p->coding == PA_SSC) {
+      cands.push_back({0, nt, 0});
+    } else {
+      for (int sw = p->msbits; sw >= 8; sw /= 2) {
+        if ((size_t)p->L * p->L * (sw / 8) + 2048 > kLdsLimit) continue;
+        cands.push_back({sw, nt, -1});
+        cands.push_back({sw, nt, 1});
+      }
+    }
+  }
+  double best = 1e300;
+  pa_params bestp = *p;
+  for (const auto& c : cands) {
+    pa_params q = *p;
+    q.slice_bits = c[0];
+    q.threads = c[1];
+    q.balance = c[2];
+    q.max_steps = 2;
+    q.trace = 0;
+    pa_run* r = nullptr;
+    rc = pa_run_create(&q, &r);
+    if (rc) continue;  // variant not valid for this shape
+    double ms = 1e300;
+    if (pa_run_step(r, 1) == PA_OK && pa_run_sync(r) == PA_OK) {
+      r->batch.timing = true;
+      r->batch.sweep_ms = 0;
+      if (pa_run_step(r, 1) == PA_OK && pa_run_sync(r) == PA_OK) ms = r->batch.sweep_ms;
+    }
+    pa_run_destroy(r);
+    if (ms < best) {
+      best = ms;
+      bestp = q;
+    }
+  }
+  if (best >= 1e299) return fail(PA_ERR_INVALID, "no sweep variant ran for this shape");
+  *tuned = *p;
+  tuned->slice_bits = bestp.slice_bits;
+  tuned->threads = bestp.threads;
+  tuned->balance = bestp.balance;
+  tune_cache()[tune_key(*p)] = TuneVal{bestp.slice_bits, bestp.threads, bestp.balance, best};
+  if (best_sweep_ms) *best_sweep_ms = best;
+  return PA_OK;
+}
+
+int pa_autotune_clear(void) {
+  tune_cache().clear();
+  return PA_OK;
 }
 
 // ---------------- batch ----------------

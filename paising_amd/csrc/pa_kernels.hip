@@ -515,7 +515,8 @@ __global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict_
   RunState* st = d.st;
   if (!st->active || st->overflow) return;
   if (d.coding != kMSC || d.w != w) return;
-  const int c = blockIdx.x - off[r];
+  const int cb = blockIdx.x - off[r];
+  const int c = cb / d.rs_split, js = cb % d.rs_split;
   const int S = d.rs_sites, N = d.N;
   const int s0 = c * S;
   if (s0 >= N) return;
@@ -531,7 +532,7 @@ __global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict_
     if (s < ns) old[i] = src[(size_t)g * N + s0 + s];
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < rgn; j += blockDim.x) {
+  for (int j = js * blockDim.x + threadIdx.x; j < rgn; j += d.rs_split * blockDim.x) {
     // parents of the w lanes of new group j: independent loads issued together
     int pl[w];
 #pragma unroll
@@ -698,41 +699,27 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
   for (int i = threadIdx.x; i < N; i += NT) g[i] = lat8[i];
 }
 
-// MSC: one block per (group, slice).  A slice holds SW = 8*sizeof(S) consecutive replica
-// bits of every site word; its LCG chain starts at ran_{q*SW} = A_{q SW} ran_0 + C_{q SW}.
-// Slices let lattices too large for LDS (L=256 x 64 bits = 512 KiB) and populations with
-// few groups use the whole chip; they do not change any stream.
-template <typename W, typename S, int NT, bool INIT>
-__global__ __launch_bounds__(NT) void k_sweep_msc(const RunDesc* __restrict__ runs,
-                                                  const int32_t* __restrict__ offs, int nruns) {
+// MSC sweeps of one unit = (group g, slice qs): sweeps [s_begin, s_end) of the step, with the
+// lattice slice staged in LDS.  A slice holds SW = 8*sizeof(S) consecutive replica bits of
+// every site word; its LCG chain starts at ran_{q*SW} = A_{q SW} ran_0 + C_{q SW}.  Slices
+// let lattices too large for LDS (L=256 x 64 bits = 512 KiB) fit, and do not change any
+// stream.  init: the freshly initialised population (buffer 0, energies only, no store).
+template <typename W, typename S, int NT>
+__device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const RunState* st, int g, int qs, uint32_t s_begin,
+                         uint32_t s_end, bool init, bool energy, S* lat, int* accE, int* accM) {
   constexpr int w = (int)sizeof(W) * 8;
   constexpr int SW = (int)sizeof(S) * 8;
-  constexpr int NS = w / SW;  // slices per word
-  const int* off = offs + kKUnits * (nruns + 1);
-  const int bu = blockIdx.x / NS;
-  const int qs = blockIdx.x % NS;
-  const int r = find_run(off, nruns, bu);
-  const RunDesc& d = runs[r];
-  RunState* st = d.st;
-  if (!INIT && (!st->active || st->overflow)) return;
-  const int g = bu - off[r];
-  const int Rn = INIT ? st->R : st->Rnew;
-  const int rgn = ceil_div(Rn, w);
-  if (g >= rgn) return;
-  extern __shared__ __align__(16) uint8_t smem_sw[];
-  S* lat = (S*)smem_sw;
-  __shared__ int accE[SW], accM[SW];
   const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
   const int step = st->step;
-  uint8_t* gbase = (uint8_t*)d.pop[INIT ? 0 : st->buf ^ 1] + (size_t)g * N * sizeof(W) +
+  uint8_t* gbase = (uint8_t*)d.pop[init ? 0 : st->buf ^ 1] + (size_t)g * N * sizeof(W) +
                    qs * sizeof(S);
+  __syncthreads();  // LDS reuse across units
   for (int i = threadIdx.x; i < N; i += NT) lat[i] = *(const S*)(gbase + (size_t)i * sizeof(W));
-  const uint32_t T4 = INIT ? 0u : (uint32_t)d.thr[3 * step + 0];
-  const uint32_t T8 = INIT ? 0u : (uint32_t)d.thr[3 * step + 1];
+  const uint32_t T4 = init ? 0u : (uint32_t)d.thr[3 * step + 0];
+  const uint32_t T8 = init ? 0u : (uint32_t)d.thr[3 * step + 1];
   const uint64_t sub0 =
       st->IS + (uint64_t)ceil_div(st->R, w) * w + (uint64_t)g * kStreamsPerUnit;
   const uint64_t seed = d.seed;
-  const uint32_t nsw = INIT ? 0u : d.eqsweeps;
   uint32_t JA = 1u, JC = 0u;
   lcg_jump(qs * SW, JA, JC);
   __syncthreads();
@@ -754,23 +741,23 @@ __global__ __launch_bounds__(NT) void k_sweep_msc(const RunDesc* __restrict__ ru
     if (nb <= P) {
       // at most one block per half sweep per thread: its four sites stay in registers
       const bool act = part < nb;
-      Site qs[2][4];
+      Site qs4[2][4];
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj)
-          qs[h][jj] = site_at(t + kStreamsPerUnit * (4 * part + jj), h, d);
-      for (uint32_t s = 0; s < nsw; ++s) {
+          qs4[h][jj] = site_at(t + kStreamsPerUnit * (4 * part + jj), h, d);
+      for (uint32_t s = s_begin; s < s_end; ++s) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           if (act)
-            block4(lat + h * Nh, lat + (h ^ 1) * Nh, qs[h],
+            block4(lat + h * Nh, lat + (h ^ 1) * Nh, qs4[h],
                    stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + part));
           __syncthreads();
         }
       }
     } else {
-      for (uint32_t s = 0; s < nsw; ++s) {
+      for (uint32_t s = s_begin; s < s_end; ++s) {
         for (int h = 0; h < 2; ++h) {
           for (int m = part; m < nb; m += P) {
             const int base = t + kStreamsPerUnit * 4 * m;
@@ -790,7 +777,7 @@ __global__ __launch_bounds__(NT) void k_sweep_msc(const RunDesc* __restrict__ ru
     const int nt = has ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
     U4 rb{0, 0, 0, 0};
     uint64_t cached = ~0ull;
-    for (uint32_t s = 0; s < nsw; ++s) {
+    for (uint32_t s = s_begin; s < s_end; ++s) {
       for (int h = 0; h < 2; ++h) {
         S* own = lat + h * Nh;
         const S* oth = lat + (h ^ 1) * Nh;
@@ -808,9 +795,120 @@ __global__ __launch_bounds__(NT) void k_sweep_msc(const RunDesc* __restrict__ ru
       }
     }
   }
-  msc_energy_slice<S>(lat, L, Ld2, Nh, accE, accM, qs * SW, d.IE, d.M, (size_t)g * w);
-  if (!INIT)
+  if (energy) msc_energy_slice<S>(lat, L, Ld2, Nh, accE, accM, qs * SW, d.IE, d.M, (size_t)g * w);
+  if (!init) {
+    __syncthreads();
     for (int i = threadIdx.x; i < N; i += NT) *(S*)(gbase + (size_t)i * sizeof(W)) = lat[i];
+  }
+}
+
+// MSC: one block per (group, slice), all EqSw sweeps of the step.
+template <typename W, typename S, int NT, bool INIT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep_msc(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns) {
+  constexpr int w = (int)sizeof(W) * 8;
+  constexpr int SW = (int)sizeof(S) * 8;
+  constexpr int NS = w / SW;  // slices per word
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int bu = blockIdx.x / NS;
+  const int qs = blockIdx.x % NS;
+  const int r = find_run(off, nruns, bu);
+  const RunDesc& d = runs[r];
+  const RunState* st = d.st;
+  if (!INIT && (!st->active || st->overflow)) return;
+  const int g = bu - off[r];
+  const int Rn = INIT ? st->R : st->Rnew;
+  if (g >= ceil_div(Rn, w)) return;
+  extern __shared__ __align__(16) uint8_t smem_sw[];
+  __shared__ int accE[SW], accM[SW];
+  msc_unit<W, S, NT>(d, st, g, qs, 0, INIT ? 0u : d.eqsweeps, INIT, true, (S*)smem_sw, accE, accM);
+}
+
+// Makespan-balanced MSC sweeps: P co-resident workers split the step's work line
+// (units x sweeps, weighted by lattice size) into equal shares, McNaughton's wrap-around
+// rule.  A unit cut by a share boundary at sweep k runs sweeps [0, k) first, at the start of
+// the later worker, and sweeps [k, E) last, at the end of the earlier worker; the lattice is
+// handed over through HBM with an agent-scope release/acquire flag tagged with the step.
+// Since a unit's work never exceeds a share, its two parts never overlap in time and the
+// wait is normally empty; it is bounded, and a timeout is reported as an error.
+__device__ __forceinline__ int64_t run_units(const RunDesc& d, int NS) {
+  const RunState* st = d.st;
+  if (!st->active || st->overflow) return 0;
+  return (int64_t)ceil_div(st->Rnew, d.w) * NS;
+}
+
+template <typename W, typename S, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_msc_bal(const RunDesc* __restrict__ runs,
+                                                      int nruns, int P) {
+  constexpr int SW = (int)sizeof(S) * 8;
+  constexpr int NS = (int)sizeof(W) * 8 / SW;
+  extern __shared__ __align__(16) uint8_t smem_sw[];
+  __shared__ int accE[SW], accM[SW];
+  S* lat = (S*)smem_sw;
+  const int wk = blockIdx.x;
+  int64_t Ctot = 0, Umax = 0;
+  for (int r = 0; r < nruns; ++r) {
+    const int64_t uc = (int64_t)runs[r].eqsweeps * runs[r].N;
+    const int64_t n = run_units(runs[r], NS);
+    Ctot += n * uc;
+    if (n > 0 && uc > Umax) Umax = uc;
+  }
+  if (Ctot == 0) return;
+  // a share never smaller than the largest unit, so a unit is cut at most once
+  const int64_t Sh = max((Ctot + P - 1) / P, Umax);
+  const int64_t a = (int64_t)wk * Sh, b = min(a + Sh, Ctot);
+  if (a >= b) return;
+  __shared__ int timed_out;
+  int64_t base = 0;
+  for (int r = 0; r < nruns && base < b; ++r) {
+    const RunDesc& d = runs[r];
+    const int64_t E = d.eqsweeps, uc = E * d.N;
+    const int64_t n = run_units(d, NS);
+    const int64_t cr = n * uc;
+    if (base + cr <= a) { base += cr; continue; }
+    const int64_t u0 = a > base ? (a - base) / uc : 0;
+    const int64_t u1 = min(n - 1, (b - 1 - base) / uc);
+    RunState* st = d.st;
+    const uint32_t tag = (uint32_t)st->step;
+    for (int64_t u = u0; u <= u1; ++u) {
+      const int64_t p0 = base + u * uc;
+      const int g = (int)(u / NS), qs = (int)(u % NS);
+      uint32_t* flag = d.flags + (size_t)g * NS + qs;
+      // task: sweeps [s0, s1); publish = hand the lattice to the earlier share afterwards;
+      // wait = take it over from the later share first
+      int64_t s0 = 0, s1 = E;
+      bool publish = false, wait = false;
+      if (p0 < a) {  // first sweeps of a unit that began in the previous share
+        const int64_t kprev = (a - p0 + d.N / 2) / d.N;
+        s1 = E - kprev;
+        publish = kprev > 0;
+      } else if (p0 + uc > b) {  // last sweeps of a unit that continues in the next share
+        const int64_t kme = (b - p0 + d.N / 2) / d.N;
+        s0 = E - kme;
+        wait = kme < E;
+      }
+      if (s1 <= s0) continue;
+      if (wait) {
+        if (threadIdx.x == 0) {
+          timed_out = 0;
+          uint32_t spins = 0;
+          while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != tag) {
+            __builtin_amdgcn_s_sleep(8);
+            if (++spins > (1u << 26)) { timed_out = 1; break; }
+          }
+          if (timed_out) atomicOr(&st->err, 1);
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      msc_unit<W, S, NT>(d, st, g, qs, (uint32_t)s0, (uint32_t)s1, false, !publish, lat, accE, accM);
+      if (publish) {
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(flag, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    base += cr;
+  }
 }
 
 // ============================================================================================
@@ -967,6 +1065,31 @@ static hipError_t launch_sweep_msc(int w, int sw, int nt, int nblocks, size_t ld
   return hipGetLastError();
 }
 
+static hipError_t launch_sweep_msc_bal(int w, int sw, int nt, int P, size_t lds, hipStream_t s,
+                                       const RunDesc* runs, int nruns) {
+#define PA_MSCB(WT, ST, NTV) \
+  hipLaunchKernelGGL((k_sweep_msc_bal<WT, ST, NTV>), dim3(P), dim3(NTV), lds, s, runs, nruns, P)
+#define PA_MSCB_NT(WT, ST)                     \
+  do {                                         \
+    if (nt == 512) PA_MSCB(WT, ST, 512);        \
+    else if (nt == 256) PA_MSCB(WT, ST, 256);   \
+    else PA_MSCB(WT, ST, 128);                  \
+  } while (0)
+  if (w == 32) {
+    if (sw == 32) PA_MSCB_NT(uint32_t, uint32_t);
+    else if (sw == 16) PA_MSCB_NT(uint32_t, uint16_t);
+    else PA_MSCB_NT(uint32_t, uint8_t);
+  } else {
+    if (sw == 64) PA_MSCB_NT(uint64_t, uint64_t);
+    else if (sw == 32) PA_MSCB_NT(uint64_t, uint32_t);
+    else if (sw == 16) PA_MSCB_NT(uint64_t, uint16_t);
+    else PA_MSCB_NT(uint64_t, uint8_t);
+  }
+#undef PA_MSCB_NT
+#undef PA_MSCB
+  return hipGetLastError();
+}
+
 template <typename F>
 static hipError_t allow_lds(F* f) {
   return hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1005,6 +1128,9 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW(k_sweep_ssc<256>);
   PA_ALLOW(k_sweep_ssc<512>);
 #define PA_ALLOW_MSC(WT, ST)                          \
+  PA_ALLOW(k_sweep_msc_bal<WT, ST, 128>);             \
+  PA_ALLOW(k_sweep_msc_bal<WT, ST, 256>);             \
+  PA_ALLOW(k_sweep_msc_bal<WT, ST, 512>);             \
   PA_ALLOW(k_sweep_msc<WT, ST, 128, false>);          \
   PA_ALLOW(k_sweep_msc<WT, ST, 256, false>);          \
   PA_ALLOW(k_sweep_msc<WT, ST, 512, false>);          \
@@ -1041,6 +1167,26 @@ hipError_t pa_launch_init(int coding, int w, int sw, int nt, int nblocks, size_t
   return launch_sweep_msc<true>(w, sw, nt, nblocks * (w / sw), lds, s, runs, offs, nruns);
 }
 
+hipError_t pa_occupancy(int kind, int coding, int w, int sw, int nt, size_t lds, int* blocks_per_cu) {
+  if (kind != 11) return hipErrorInvalidValue;
+  (void)coding;
+#define PA_OCC(WT, ST, NTV) \
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_sweep_msc_bal<WT, ST, NTV>, NTV, lds)
+#define PA_OCC_NT(WT, ST) do { if (nt == 512) PA_OCC(WT, ST, 512); else if (nt == 256) PA_OCC(WT, ST, 256); else PA_OCC(WT, ST, 128); } while (0)
+  if (w == 32) {
+    if (sw == 32) PA_OCC_NT(uint32_t, uint32_t);
+    else if (sw == 16) PA_OCC_NT(uint32_t, uint16_t);
+    else PA_OCC_NT(uint32_t, uint8_t);
+  } else {
+    if (sw == 64) PA_OCC_NT(uint64_t, uint64_t);
+    else if (sw == 32) PA_OCC_NT(uint64_t, uint32_t);
+    else if (sw == 16) PA_OCC_NT(uint64_t, uint16_t);
+    else PA_OCC_NT(uint64_t, uint8_t);
+  }
+#undef PA_OCC_NT
+#undef PA_OCC
+}
+
 hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nblocks, size_t lds,
                           hipStream_t s, const RunDesc* runs, const int32_t* offs, int nruns) {
   if (nblocks <= 0) return hipSuccess;
@@ -1063,6 +1209,7 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 7: hipLaunchKernelGGL(k_fin, dim3(nblocks), dim3(64), 0, s, runs, nruns); break;
     case 8: hipLaunchKernelGGL(k_qgroup, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 9: hipLaunchKernelGGL(k_parents, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
+    case 11: return launch_sweep_msc_bal(w, sw, nt, nblocks, lds, s, runs, nruns);
     case 10:
       if (w == 32) hipLaunchKernelGGL(k_resample_msc<uint32_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
       else hipLaunchKernelGGL(k_resample_msc<uint64_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);

@@ -37,3 +37,28 @@ def test_uniform_conventions(uniform):
     orows = oracle_rows(*args, uniform=uniform)
     erows, _ = pa.run_all(engine_params(*args, uniform=uniform))
     compare_rows(orows, erows)
+
+
+@pytest.mark.parametrize("L,w,R,eqs,steps", [(64, 32, 20000, 20, 3), (32, 32, 9000, 7, 4),
+                                            (64, 64, 30000, 5, 3), (16, 32, 3000, 9, 3)])
+def test_balanced_sweeps_match_unbalanced(L, w, R, eqs, steps):
+    """Makespan-balanced MSC sweeps (units cut across workers) give the same bits."""
+    ref, _ = pa.run_all(engine_params(L, pa.MSC, w, R, eqs, 0.005, 1.0, 5, steps, balance=-1))
+    got, _ = pa.run_all(engine_params(L, pa.MSC, w, R, eqs, 0.005, 1.0, 5, steps, balance=1))
+    for x, y in zip(ref, got):
+        assert (x.R, x.sumE, x.sumM, x.spin_hash, x.roff_hash) == (y.R, y.sumE, y.sumM, y.spin_hash, y.roff_hash)
+        assert x.lnQ.hex() == y.lnQ.hex()
+
+
+def test_balanced_batch_of_mixed_sizes():
+    plist = [engine_params(L, pa.MSC, 32, R, e, 0.01, 1.0, 40 + i, 3, balance=1)
+             for i, (L, R, e) in enumerate([(64, 7000, 9), (32, 12000, 17), (96, 4000, 5), (64, 9000, 13)])]
+    b = pa.Batch(plist, use_graph=True)
+    b.run()
+    for i, p in enumerate(plist):
+        p.balance = -1
+        plain, _ = pa.run_all(p)
+        got = b.results(i)
+        assert [r.spin_hash for r in plain] == [r.spin_hash for r in got]
+        assert [r.lnQ.hex() for r in plain] == [r.lnQ.hex() for r in got]
+    b.close()
