@@ -99,11 +99,40 @@ __device__ __forceinline__ Site site_at(int idx, int side, const RunDesc& d) {
 
 // ---- Boltzmann acceptance, SSC ----------------------------------------------------------
 // u < T  <=>  v < V(T) for the 32-bit word v behind u; V is computed on the host per step
-// for T = 1 (dE <= 0), exp(-4 beta), exp(-8 beta).
-__device__ __forceinline__ bool ssc_accept(int prod, uint32_t v, uint64_t V1, uint64_t V4,
-                                           uint64_t V8) {
-  const uint64_t thr = prod == 4 ? V8 : (prod == 2 ? V4 : V1);
-  return (uint64_t)v < thr;
+// for T = 1 (dE <= 0), exp(-4 beta), exp(-8 beta).  The kernel keeps the three values as
+// 32-bit words in a 9-entry LDS table indexed by prod + 4, prod = s_i * sum of neighbours:
+// one LDS read instead of a compare/select chain.  V(1) can be 2^32 (midpoint mapping:
+// every u < 1); ALWAYS1 then accepts every dE <= 0 move.
+template <bool ALWAYS1>
+__device__ __forceinline__ void ssc_flip(int8_t* own, const int8_t* oth, const Site& q,
+                                         uint32_t v, const uint32_t* tab) {
+  const int sp = own[q.idx];
+  const int prod = sp * (oth[q.i1] + oth[q.idx] + oth[q.i3] + oth[q.i4]);
+  bool acc = v < tab[prod + 4];
+  if (ALWAYS1) acc = acc || prod <= 0;
+  own[q.idx] = (int8_t)(acc ? -sp : sp);
+}
+
+// Four sites of one sublattice with the four words of one Philox block.  sp holds the
+// sites' spins (a site is only ever written by the thread that updates it, so the caller
+// may keep them in registers across sweeps).  All LDS reads are issued before the first
+// store: the stores go to the other sublattice, which the compiler cannot prove.
+template <bool ALWAYS1>
+__device__ __forceinline__ void ssc_flip4(int8_t* own, const int8_t* oth, const Site (&q)[4],
+                                          int (&sp)[4], const U4& rb, const uint32_t* tab) {
+  int nb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    nb[j] = oth[q[j].i1] + oth[q[j].idx] + oth[q[j].i3] + oth[q[j].i4];
+  const uint32_t v[4] = {rb.x, rb.y, rb.z, rb.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int prod = sp[j] * nb[j];
+    bool acc = v[j] < tab[prod + 4];
+    if (ALWAYS1) acc = acc || prod <= 0;
+    sp[j] = acc ? -sp[j] : sp[j];
+    own[q[j].idx] = (int8_t)sp[j];
+  }
 }
 
 // ---- MSC word update (MSC:207-229) -----------------------------------------------------
@@ -574,53 +603,29 @@ __global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict_
 // blocks per half sweep, and each thread evaluates one block for four sites at once.
 // ============================================================================================
 
-// SSC: one block per replica.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ runs,
-                                                  const int32_t* __restrict__ offs, int nruns) {
-  const int* off = offs + kKUnits * (nruns + 1);
-  const int r = find_run(off, nruns, blockIdx.x);
-  const RunDesc& d = runs[r];
-  RunState* st = d.st;
-  if (!st->active || st->overflow) return;
-  const int B = blockIdx.x - off[r];
-  const int Rn = st->Rnew;
-  if (B >= Rn) return;
-  extern __shared__ int8_t lat8[];
-  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
-  const int step = st->step;
-  int8_t* g = (int8_t*)d.pop[st->buf ^ 1] + (size_t)B * N;
-  for (int i = threadIdx.x; i < N; i += NT) lat8[i] = g[i];
-  const uint64_t V1 = d.thr[3 * step + 0], V4 = d.thr[3 * step + 1], V8 = d.thr[3 * step + 2];
-  const uint64_t sub0 = st->IS + (uint64_t)st->R + (uint64_t)B * kStreamsPerUnit;
-  const uint64_t seed = d.seed;
-  const uint32_t nsw = d.eqsweeps;
-  __syncthreads();
-  // thresholds as 32-bit words: accept iff v < thr, or always for dE <= 0 when V(1) = 2^32
-  const uint32_t V1w = (uint32_t)min(V1, (uint64_t)0xFFFFFFFFu);
-  const uint32_t V4w = (uint32_t)V4, V8w = (uint32_t)V8;
-  const bool always1 = V1 > 0xFFFFFFFFull;
+// SSC sweeps of one replica held in LDS (lat8: sublattice A then B).
+template <int NT, bool ALWAYS1>
+__device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint64_t seed,
+                                           uint64_t sub0, uint32_t nsw,
+                                           const uint32_t* tab) {
+  const int Nh = d.Nh;
   if ((Nh & 511) == 0) {
     // aligned: thread = (stream t, part); blocks m = part, part + P, ... of nb per half
     constexpr int P = NT / kStreamsPerUnit;
     const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
     const int nb = Nh / 512;
-    auto update = [&](int8_t* own, const int8_t* oth, const Site& q, uint32_t v) {
-      const int sp = own[q.idx];
-      const int prod = sp * (oth[q.i1] + oth[q.idx] + oth[q.i3] + oth[q.i4]);
-      const uint32_t thr = prod == 4 ? V8w : (prod == 2 ? V4w : V1w);
-      const bool acc = (v < thr) || (always1 && prod <= 0);
-      own[q.idx] = (int8_t)(acc ? -sp : sp);
-    };
     if (nb <= P) {
       // at most one block per half sweep per thread: its four sites stay in registers
       const bool act = part < nb;
       Site qs[2][4];
+      int sp[2][4];
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj)
+        for (int jj = 0; jj < 4; ++jj) {
           qs[h][jj] = site_at(t + kStreamsPerUnit * (4 * part + jj), h, d);
+          sp[h][jj] = act ? lat8[h * Nh + qs[h][jj].idx] : 0;
+        }
       for (uint32_t s = 0; s < nsw; ++s) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -628,10 +633,7 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
             int8_t* own = lat8 + h * Nh;
             const int8_t* oth = lat8 + (h ^ 1) * Nh;
             const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + part);
-            update(own, oth, qs[h][0], rb.x);
-            update(own, oth, qs[h][1], rb.y);
-            update(own, oth, qs[h][2], rb.z);
-            update(own, oth, qs[h][3], rb.w);
+            ssc_flip4<ALWAYS1>(own, oth, qs[h], sp[h], rb, tab);
           }
           __syncthreads();
         }
@@ -644,10 +646,11 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
           for (int m = part; m < nb; m += P) {
             const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
             const int base = t + kStreamsPerUnit * 4 * m;
-            update(own, oth, site_at(base, h, d), rb.x);
-            update(own, oth, site_at(base + kStreamsPerUnit, h, d), rb.y);
-            update(own, oth, site_at(base + 2 * kStreamsPerUnit, h, d), rb.z);
-            update(own, oth, site_at(base + 3 * kStreamsPerUnit, h, d), rb.w);
+            const Site q[4] = {site_at(base, h, d), site_at(base + kStreamsPerUnit, h, d),
+                               site_at(base + 2 * kStreamsPerUnit, h, d),
+                               site_at(base + 3 * kStreamsPerUnit, h, d)};
+            int sp[4] = {own[q[0].idx], own[q[1].idx], own[q[2].idx], own[q[3].idx]};
+            ssc_flip4<ALWAYS1>(own, oth, q, sp, rb, tab);
           }
           __syncthreads();
         }
@@ -667,18 +670,46 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
         for (int k = 0; k < nt; ++k) {
           const uint64_t p = (uint64_t)(2 * s + h) * nt + k;
           if ((p >> 2) != cached) { cached = p >> 2; rb = stream_block(seed, sub0 + t, cached); }
-          const uint32_t v = u4_word(rb, (int)(p & 3));
-          const Site q = site_at(t + kStreamsPerUnit * k, h, d);
-          const int sp = own[q.idx];
-          const int prod = sp * (oth[q.i1] + oth[q.idx] + oth[q.i3] + oth[q.i4]);
-          const uint32_t thr = prod == 4 ? V8w : (prod == 2 ? V4w : V1w);
-          const bool acc = (v < thr) || (always1 && prod <= 0);
-          own[q.idx] = (int8_t)(acc ? -sp : sp);
+          ssc_flip<ALWAYS1>(own, oth, site_at(t + kStreamsPerUnit * k, h, d),
+                            u4_word(rb, (int)(p & 3)), tab);
         }
         __syncthreads();
       }
     }
   }
+}
+
+// SSC: one block per replica.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int B = blockIdx.x - off[r];
+  const int Rn = st->Rnew;
+  if (B >= Rn) return;
+  extern __shared__ int8_t lat8[];
+  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
+  const int step = st->step;
+  int8_t* g = (int8_t*)d.pop[st->buf ^ 1] + (size_t)B * N;
+  for (int i = threadIdx.x; i < N; i += NT) lat8[i] = g[i];
+  const uint64_t V1 = d.thr[3 * step + 0], V4 = d.thr[3 * step + 1], V8 = d.thr[3 * step + 2];
+  const uint64_t sub0 = st->IS + (uint64_t)st->R + (uint64_t)B * kStreamsPerUnit;
+  __shared__ uint32_t tab[9];
+  if (threadIdx.x < 9) {
+    const int prod = (int)threadIdx.x - 4;
+    tab[threadIdx.x] = prod == 4 ? (uint32_t)V8
+                     : prod == 2 ? (uint32_t)V4
+                                 : (uint32_t)min(V1, (uint64_t)0xFFFFFFFFu);
+  }
+  __syncthreads();
+  if (V1 > 0xFFFFFFFFull)
+    ssc_sweeps<NT, true>(lat8, d, d.seed, sub0, d.eqsweeps, tab);
+  else
+    ssc_sweeps<NT, false>(lat8, d, d.seed, sub0, d.eqsweeps, tab);
   // energy and magnetisation (energyKer SSC:195-215)
   int e = 0, mm = 0;
   for (int idx = threadIdx.x; idx < Nh; idx += NT) {
