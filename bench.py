@@ -8,10 +8,13 @@ averages) with the population resident in HBM.  The metric is spin-flip attempts
 EqSw * N * sum_i R_i / t, the reciprocal of the reference's "Time per spin-flip"
 (PAisingMSC.cu:957).
 
-Extra fields: the sweep kernel's VALU roofline (attempts x VALU issue slots per attempt vs
-the chip's issue rate), the resampling kernel's HBM roofline, and a CPU baseline (the
-oracle, OpenMP, on this host) over a bounded sample.  --mode batch / serve measure the
-batched (configs[3]-style mix) and served (Poisson arrivals) paths instead.
+Extra fields: the sweep kernel's VALU issue roofline ("roofline": algorithmic VALU lane-ops
+per attempt x attempts / summed sweep-kernel time, against CUs x 64 lanes x clock), the
+resampling kernel's HBM roofline ("roofline_resample"), the same run in single-spin coding
+("ssc") and a CPU baseline (the oracle, C++/OpenMP, on this host's cores) over a bounded
+sample.  Kernel times come from HIP events recorded on the run's stream inside the timed
+region.  --mode batch / serve measure the batched (64-run mix of uneven shapes) and served
+(Poisson arrivals) paths instead.
 """
 import argparse
 import json
@@ -28,25 +31,33 @@ import paising_amd as pa  # noqa: E402
 L, RINIT, EQSW, DB, SEED = 64, 20000, 100, 0.005, 100
 N = L * L
 
-# VALU issue slots per spin-flip attempt of the MSC32 sweep (lane-cycles), counted on the
-# algorithm of PAisingMSC.cu:207-229 in its cheapest gfx950 form: per replica bit one LCG
-# value (v_mul_lo_u32 + v_add_u32, jump-ahead form) and two threshold tests each a
-# compare plus a bit insert; per 32-bit word a quarter of a Philox4x32-10 block (10 rounds
-# x 2 v_mad_u64_u32 + 2 v_xor3) and the neighbour adder/acceptance logic (17 bitwise ops).
-# MUL_SLOTS is the issue cost of one v_mul_lo_u32 / v_mad_u64_u32 in VALU cycles relative
-# to v_add_u32 (measured with _work-style microbenchmarks on the box).
-MUL_SLOTS = float(os.environ.get("PA_MUL_SLOTS", "1.0"))
+# VALU roofline of the sweeps.  The sweep kernels run from LDS and issue only VALU, SALU and
+# LDS instructions; on gfx950 a wave64 VALU instruction occupies its SIMD for 4 cycles
+# (16 lanes per cycle), v_mul_lo_u32 / v_mad_u64_u32 included, so the issue peak is
+# CUs x 4 SIMD x 16 lanes x clock lane-ops/s.  Algorithmic VALU ops per spin-flip attempt, in
+# the cheapest gfx950 form of PAisingMSC.cu:207-229 and PAisingSSC.cu:164-188:
+#   MSC, per replica bit: one LCG step (v_mad_u64_u32) and two threshold tests, each a
+#     compare into a lane mask plus an add-with-carry that shifts the bit in (5 ops);
+#   MSC, per w-bit word: a quarter Philox4x32-10 block (10 rounds x {2 v_mad_u64_u32,
+#     2 v_bitop3} / 4 = 10), the bit-sliced neighbour count and acceptance (15 bitwise ops)
+#     and two bit reversals (27 ops);
+#   SSC, per site: a quarter Philox block (10), neighbour sum (2), spin product (1), table
+#     index (1), compare (1), select and negate (2): 17 ops.
+MSC_OPS_PER_BIT, MSC_OPS_PER_WORD, SSC_OPS_PER_SITE = 5, 27, 17
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# workload (FETCH_SIZE scaled x2, WRITE_SIZE x1, the factors a plain device copy of known
+# size reads back); summaries in profiles/.
+TRAFFIC_BYTES = {"sweep_msc32": None, "resample_msc32": None}
 
 
-def valu_slots_per_attempt(w=32):
-    per_bit = MUL_SLOTS + 1 + 2 * 2
-    philox_block = 10 * (2 * MUL_SLOTS + 2)
-    per_word = philox_block / 4 + 17
-    return per_bit + per_word / w
+def msc_ops_per_attempt(w=32):
+    return MSC_OPS_PER_BIT + MSC_OPS_PER_WORD / w
 
 
-def peak_valu_slots_per_s(cu, clock_hz):
-    return cu * 64 * clock_hz  # 4 SIMD x 16 lanes per CU per cycle
+def peak_lane_ops_per_s(cu, clock_hz):
+    return cu * 64 * clock_hz
 
 
 def device_clock_hz():
@@ -62,55 +73,64 @@ def device_clock_hz():
     return 2.4e9
 
 
-def cpu_baseline(sample_steps=1, threads=None):
-    """Oracle (host C++/OpenMP restatement of the reference) on a bounded sample of the same
-    workload: the first `sample_steps` beta steps of config 2, timed on this host."""
+def cpu_threads():
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(sample_steps, threads):
+    """The oracle (host C++/OpenMP restatement of the reference kernels) on a bounded sample
+    of the same workload: initial population plus the first `sample_steps` beta steps of
+    the L=64 R=20000 MSC32 run, timed on this host."""
     import oracle
-    threads = threads or os.cpu_count() or 1
     rc, runs, el, tot = oracle.run(L=L, coding=oracle.MSC, msbits=32, Rinit=RINIT, eqsweeps=EQSW,
                                    dB=DB, Bfin=1.0, seed=SEED, max_steps=sample_steps,
                                    threads=threads)
     assert rc == 0
     att = EQSW * N * tot
-    return {"value": att / (el * 1e9), "unit": "spin-flip attempts/ns", "cores": threads,
+    return {"value": round(att / (el * 1e9), 4), "unit": "spin-flip attempts/ns", "cores": threads,
             "kind": "port",
-            "sample": f"oracle, first {sample_steps} beta step(s) of L=64 R=20000 EqSw=100 MSC32, "
-                      f"{el:.1f} s wall on {threads} OpenMP threads"}
+            "sample": f"oracle (C++/OpenMP), init + first {sample_steps} beta steps of L=64 R=20000 "
+                      f"EqSw=100 MSC32, {el:.1f} s wall on {threads} threads"}
 
 
-def bench_plain(steps, warmup):
+def timed_run(coding, steps, warmup, tune=True):
+    """Plain path at the reference sample shape: warmup steps, then `steps` steps timed on
+    the host around the launches + one sync, with per-step kernel events on the run's
+    stream.  Returns attempts, wall s, kernel times, tuned variant and rows."""
     total = pa.schedule_steps(0.0, DB, 1.0)
     if warmup + steps > total:
         raise SystemExit(f"--warmup + --steps must be <= {total}")
-    p = pa.make_params(L=L, coding=pa.MSC, msbits=32, Rinit=RINIT, eqsweeps=EQSW, dB=DB, Bfin=1.0,
+    p = pa.make_params(L=L, coding=coding, msbits=32, Rinit=RINIT, eqsweeps=EQSW, dB=DB, Bfin=1.0,
                        seed=SEED)
+    if tune:
+        p, _ = pa.autotune(p)  # picks slice width / block shape / balancing; outside the timing
     run = pa.Run(p)
     run.step(warmup)
     run.sync()
-    rows0 = run.results()
+    run.set_timing(True)
     t0 = time.perf_counter()
     run.step(steps)
     run.sync()
-    t1 = time.perf_counter()
+    wall = time.perf_counter() - t0
+    rs_ms, sw_ms, nsteps = run.kernel_times()
     rows = run.results()
-    timed = rows[warmup + 1: warmup + 1 + steps]
-    attempts = EQSW * N * sum(r.R for r in timed)
-    wall = t1 - t0
-    # sweep-kernel timing with HIP events on the run's stream (separate pass, same steps)
-    run2 = pa.Run(p)
-    run2.step(warmup)
-    run2.sync()
-    run2.set_timing(True)
-    run2.step(steps)
-    run2.sync()
-    sweep_ms, _ = run2.timing()
     run.close()
-    run2.close()
-    return attempts, wall, sweep_ms, rows0, rows
+    timed = rows[warmup + 1: warmup + 1 + steps]
+    assert nsteps == steps and len(timed) == steps
+    attempts = EQSW * N * sum(r.R for r in timed)
+    # resampling reads the parents and writes the children: N bits (MSC) or bytes (SSC) each
+    bpr = N / 8 if coding == pa.MSC else N
+    rs_bytes = sum(bpr * (r.Rprev + r.R) for r in timed)
+    return dict(attempts=attempts, wall=wall, sweep_ms=sw_ms, resample_ms=rs_ms, rs_bytes=rs_bytes,
+                variant={"slice_bits": p.slice_bits, "threads": p.threads, "balance": p.balance},
+                rows=rows)
 
 
 def bench_config3(args):
-    """MSC 64 replicas/uint64, L=256, R=200000, EqSw=200, dB=0.002, timed on
+    """At-scale MSC: 64 replicas/uint64, L=256, R=200000, EqSw=200, dB=0.002, timed on
     the first --steps beta steps after --warmup (per-attempt work does not depend on beta)."""
     p = pa.make_params(L=256, coding=pa.MSC, msbits=64, Rinit=200000, eqsweeps=200, dB=0.002,
                        Bfin=1.0, seed=100, max_steps=args.warmup + args.steps)
@@ -124,14 +144,14 @@ def bench_config3(args):
     rows = run.results()
     run.close()
     att = 200 * 65536 * sum(r.R for r in rows[args.warmup + 1:])
-    return {"metric": "spin-flip attempts/ns, L=256 R=2e5 MSC64 (configs[2], truncated schedule)",
+    return {"metric": "spin-flip attempts/ns, L=256 R=2e5 MSC64 (truncated schedule)",
             "value": round(att / (wall * 1e9), 2), "unit": "spin-flip attempts/ns",
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 2),
             "R": [r.R for r in rows]}
 
 
 def batch_mix(nruns=64, seed=2026, steps=0):
-    """independent runs of uneven L, R and EqSw, both codings, dB=0.005, beta 0->1."""
+    """The batch mix: 64 independent runs of uneven L, R and EqSw, both codings, dB=0.005, beta 0->1."""
     import random
     rng = random.Random(seed)
     plist = []
@@ -163,7 +183,7 @@ def bench_batch(args):
                 [r.spin_hash for r in x] == [r.spin_hash for r in y] for x, y in zip(brows, prows))
     steps = sum(len(r) - 1 for r in brows)
     return {
-        "metric": "batched PA runs/s (configs[3] mix) vs plain one-at-a-time",
+        "metric": "batched PA runs/s (64-run mix) vs plain one-at-a-time",
         "value": round(len(plist) / t_batch, 4), "unit": "runs/s", "n_gpus": 1,
         "higher_is_better": True, "plain_runs_per_s": round(len(plist) / t_plain, 4),
         "speedup_vs_plain": round(t_plain / t_batch, 3), "bit_exact_vs_plain": exact,
@@ -176,7 +196,7 @@ def bench_batch(args):
 
 
 def serve_requests(n, seed=5):
-    """Poisson arrivals (20/s), L in {16..256 step 16}, R log-uniform in [1e3,1e5],
+    """Served load: Poisson arrivals (20/s), L in {16..256 step 16}, R log-uniform in [1e3,1e5],
     dB in {0.001, 0.005}, Bfin uniform in [0.44, 1.0], EqSw=50, MSC32."""
     import random
     rng = random.Random(seed)
@@ -217,7 +237,7 @@ def bench_serve(args):
             exact &= pa.format_rows(srv.results(t), 17) == pa.format_rows(pa.run_all(p)[0], 17)
     srv.close()
     return {
-        "metric": "served PA runs: p50/p99 completion latency (configs[4] arrivals)",
+        "metric": "served PA runs: p50/p99 completion latency (Poisson 20 req/s arrivals)",
         "value": round(pct(0.5), 4), "unit": "s (p50 latency)", "p99_s": round(pct(0.99), 4),
         "runs_per_s": round(len(lat) / wall, 4), "n_requests": len(lat), "n_gpus": 1,
         "higher_is_better": False, "bit_exact_vs_plain_checked": args.check if exact else "FAILED",
@@ -233,6 +253,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "copy", "config3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=6, help="beta steps in the CPU baseline sample")
+    ap.add_argument("--no-ssc", action="store_true", help="skip the SSC line")
     ap.add_argument("--nruns", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=0, help="truncate batched runs (0 = full)")
     ap.add_argument("--nreq", type=int, default=16)
@@ -241,7 +263,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs each PA population on a single "
-              "device (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "device; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.mode == "batch":
         print(json.dumps(bench_batch(args)), flush=True)
@@ -259,11 +281,12 @@ def main():
         return
     name, cu, _ = pa.device_info()
     clock = device_clock_hz()
-    attempts, wall, sweep_ms, rows0, rows = bench_plain(args.steps, args.warmup)
-    value = attempts / (wall * 1e9)
-    slots = valu_slots_per_attempt(32)
-    peak = peak_valu_slots_per_s(cu, clock)
-    achieved = attempts * slots / (sweep_ms * 1e-3)
+    peak = peak_lane_ops_per_s(cu, clock)
+    m = timed_run(pa.MSC, args.steps, args.warmup)
+    value = m["attempts"] / (m["wall"] * 1e9)
+    ops = msc_ops_per_attempt(32)
+    achieved = m["attempts"] * ops / (m["sweep_ms"] * 1e-3)
+    rs_gbs = m["rs_bytes"] / (m["resample_ms"] * 1e-3) / 1e9
     out = {
         "metric": "spin-flip attempts/ns at L=64 R=2e4 (plain MSC32 PA step)",
         "value": round(value, 4),
@@ -271,23 +294,41 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "ms_per_step": round(m["wall"] * 1e3 / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded Philox initial spins, reference schedule)",
         "config": {"workload": "plain PA, MSC 32 replicas/uint32, L=64, R=20000, EqSw=100, "
-                               "dB=0.005, beta 0->1, seed 100",
-                   "device": name},
-        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
-                     "peak": round(peak / 1e12, 3), "unit": "Tlane-op/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "slots_per_attempt": round(slots, 3),
-                     "sweep_attempts_per_ns": round(attempts / (sweep_ms * 1e6), 4)},
+                               "dB=0.005, beta 0->1, seed 100", "device": name,
+                   "variant": m["variant"]},
+        "roofline": {"bound": "valu", "kernel": "k_sweep_msc (sweeps + energies)",
+                     "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 3),
+                     "unit": "Tlane-op/s", "frac": round(achieved / peak, 4),
+                     "traffic": TRAFFIC_BYTES["sweep_msc32"],
+                     "ops_per_attempt": round(ops, 4),
+                     "sweep_ms_per_step": round(m["sweep_ms"] / args.steps, 4),
+                     "sweep_attempts_per_ns": round(m["attempts"] / (m["sweep_ms"] * 1e6), 2)},
+        "roofline_resample": {"bound": "hbm", "kernel": "k_resample_msc",
+                              "achieved": round(rs_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(rs_gbs / HBM_PEAK_GBS, 4),
+                              "traffic": TRAFFIC_BYTES["resample_msc32"],
+                              "resample_ms_per_step": round(m["resample_ms"] / args.steps, 4)},
     }
+    if not args.no_ssc:
+        s_ = timed_run(pa.SSC, args.steps, args.warmup)
+        s_ach = s_["attempts"] * SSC_OPS_PER_SITE / (s_["sweep_ms"] * 1e-3)
+        out["ssc"] = {"value": round(s_["attempts"] / (s_["wall"] * 1e9), 4),
+                      "unit": "spin-flip attempts/ns",
+                      "workload": "plain PA, SSC int8, L=64, R=20000, EqSw=100, dB=0.005, seed 100",
+                      "ms_per_step": round(s_["wall"] * 1e3 / args.steps, 4),
+                      "sweep_attempts_per_ns": round(s_["attempts"] / (s_["sweep_ms"] * 1e6), 2),
+                      "valu_frac": round(s_ach / peak, 4),
+                      "resample_GBps": round(s_["rs_bytes"] / (s_["resample_ms"] * 1e-3) / 1e9, 1),
+                      "variant": s_["variant"]}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, cpu_threads())
     print(json.dumps(out), flush=True)
 
 

@@ -108,7 +108,12 @@ int pa_run_population(pa_run* r, void* dst, size_t* bytes);
 int pa_run_array(pa_run* r, int which, void* dst, size_t* bytes);
 /* Elapsed device time of the sweep kernels and of the whole run so far (ms). */
 int pa_run_timing(pa_run* r, double* sweep_ms, double* total_ms);
-/* Time each step's sweep launches with HIP events on the run's stream (1 = on). */
+/* Device time of the resampling kernels and of the sweep kernels summed over the timed
+ * steps, and their count (valid after pa_run_sync).  The reference times only the whole
+ * program (PAisingSSC.cu:445, 801-806). */
+int pa_run_kernel_times(pa_run* r, double* resample_ms, double* sweep_ms, int* steps);
+/* Time each step's resampling and sweep launches with HIP events on the run's stream
+ * (1 = on); the events are read at the next pa_run_sync, so timing adds no host sync. */
 int pa_run_set_timing(pa_run* r, int on);
 /* Replay steps from a captured hipGraph (1 = on). */
 int pa_run_set_graph(pa_run* r, int on);

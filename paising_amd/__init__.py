@@ -65,6 +65,7 @@ SIGNATURES = {
     "pa_run_population": (_I, [_P, _P, _P]),
     "pa_run_array": (_I, [_P, _I, _P, _P]),
     "pa_run_timing": (_I, [_P, _P, _P]),
+    "pa_run_kernel_times": (_I, [_P, _P, _P, _P]),
     "pa_run_set_timing": (_I, [_P, _I]),
     "pa_run_set_graph": (_I, [_P, _I]),
     "pa_run_destroy": (_I, [_P]),
@@ -204,6 +205,14 @@ class Run:
         t = ctypes.c_double(0)
         _check(lib().pa_run_timing(self.h, ctypes.byref(s), ctypes.byref(t)))
         return s.value, t.value
+
+    def kernel_times(self):
+        """(resample_ms, sweep_ms, steps) summed over the steps timed with set_timing(True)."""
+        a = ctypes.c_double(0)
+        b = ctypes.c_double(0)
+        n = ctypes.c_int(0)
+        _check(lib().pa_run_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
 
     def results(self, max_rows=10000):
         buf = (Row * max_rows)()

@@ -55,7 +55,7 @@ struct RunDesc {
   int32_t trace;            // compute per-step hashes
   int32_t q128, r128;       // 128 / Ld2, 128 % Ld2 (site-walk increments)
   uint32_t ld2_magic;       // floor(2^32 / Ld2) + 1: idx / Ld2 == mulhi(idx, magic) for idx < 2^17
-  int32_t pad1;
+  int32_t rs_lp;            // MSC resampling: site chunks per 128-byte line (same XCD)
   uint64_t seed;
   void* pop[2];             // spin buffers: SSC int8 [cap][N], MSC word [cap][N]
   uint64_t* amask[2];       // MSC active-lane bit masks per group (ping-pong with pop)
@@ -66,7 +66,7 @@ struct RunDesc {
   int32_t* par;             // parent of every child of the last resampling [cap_pop]
   uint32_t* flags;          // balanced sweeps: hand-over flag per (group, slice) [cap_units*8]
   double* gsum;             // MSC per-group lane sums of the q terms [cap_units]
-  int32_t rs_sites;         // MSC resampling: site words per workgroup (64-byte chunks)
+  int32_t rs_sites;         // MSC resampling: site words per workgroup chunk
   int32_t rs_split;         // MSC resampling: workgroups sharing one site chunk (new groups split)
   uint32_t* cblk;           // per-1024 chunk sums of Roff
   double* qblk;             // per-1024 chunk partial q

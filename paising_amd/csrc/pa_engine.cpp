@@ -96,6 +96,7 @@ struct RunHost {
   int sw = 0, nt = 0;
   int rs_sites = 1;
   int rs_split = 1;
+  int rs_lp = 1;
   std::vector<double> B;
   size_t footprint = 0;
   // device memory (one allocation carved into pieces)
@@ -115,7 +116,11 @@ struct RunHost {
       case kKGroupChunks: return ceil_div(cap_units, kRedBlock);
       case kKAvg: return p.coding == kSSC ? ceil_div(cap_pop, kRedBlock)
                                           : ceil_div(cap_units, kRedBlock) * w;
-      case kKResample: return p.coding == kSSC ? cap_units : ceil_div(N, rs_sites) * rs_split;
+      // multiples of 8 blocks per run keep every run's first block on XCD 0 (the dispatcher
+      // deals workgroups to the 8 XCDs round-robin); k_resample_msc relies on it
+      case kKResample:
+        return p.coding == kSSC ? ceil_div(cap_units, 8) * 8
+                                : ceil_div(ceil_div(N, rs_sites), 8 * rs_lp) * 8 * rs_lp * rs_split;
     }
     return 0;
   }
@@ -199,12 +204,15 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma) {
     if (sw > h.w) sw = h.w;
     if ((size_t)h.N * (sw / 8) + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice slice exceeds LDS");
     h.sw = sw;
-    // resampling stages a 64-byte chunk of every parent group in LDS (<= 96 KiB)
+    // resampling stages a 64-byte chunk of every parent group in LDS (<= 96 KiB); the
+    // chunks of one 128-byte line are resampled on one XCD so the line is fetched once
     h.rs_sites = 64 / (h.w / 8);
     while (h.rs_sites > 1 && (size_t)h.cap_units * h.rs_sites * (h.w / 8) > 96 * 1024) h.rs_sites /= 2;
+    h.rs_lp = 128 / (h.rs_sites * (h.w / 8));
     if ((size_t)h.cap_units * h.rs_sites * (h.w / 8) > 150 * 1024)
       return fail(PA_ERR_INVALID, "population too large for one device resampling chunk");
-    // about one new group per thread: the site chunk is re-read from L2 by each split
+    // about one new group per thread; the splits of one site chunk run on one XCD and
+    // re-read the chunk from that XCD's L2
     h.rs_split = std::max(1, std::min(16, ceil_div(h.cap_units, 256)));
   } else {
     if ((size_t)h.N + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice exceeds LDS");
@@ -256,6 +264,7 @@ int materialize(RunHost& h) {
   d.cap_units = h.cap_units; d.umap = h.p.uniform; d.eqsweeps = h.p.eqsweeps;
   d.trace = h.p.trace; d.q128 = 128 / d.Ld2; d.r128 = 128 % d.Ld2; d.seed = h.p.seed;
   d.rs_sites = h.rs_sites;
+  d.rs_lp = h.rs_lp;
   d.rs_split = h.rs_split;
   d.ld2_magic = (uint32_t)((1ull << 32) / (uint64_t)d.Ld2 + 1);
   std::vector<double> dbeta(h.n_steps + 1, 0.0);
@@ -368,8 +377,13 @@ struct Batch {
   int steps_launched = 0;
   int max_steps = 0;
   bool timing = false;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
-  double sweep_ms = 0.0, total_ms = 0.0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // timed steps: three events each (before resampling, before sweeps, after sweeps),
+  // recorded on the run's stream and read back by collect_timing() after a sync
+  std::vector<hipEvent_t> tev;
+  size_t tev_used = 0;
+  double sweep_ms = 0.0, resample_ms = 0.0, total_ms = 0.0;
+  int timed_steps = 0;
   std::vector<void*> table_allocs;
 
   ~Batch() { release(); }
@@ -384,7 +398,9 @@ struct Batch {
   void release() {
     release_tables();
     if (ev0) { (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1); ev0 = ev1 = nullptr; }
-    if (evs0) { (void)hipEventDestroy(evs0); (void)hipEventDestroy(evs1); evs0 = evs1 = nullptr; }
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+    tev.clear();
+    tev_used = 0;
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
   }
 
@@ -495,21 +511,51 @@ struct Batch {
   }
 
   // One beta step of every active run (8 launches + one sweep launch per variant).
+  int timing_event(hipEvent_t* e) {
+    if (tev_used == tev.size()) {
+      hipEvent_t x;
+      HIPCHECK(hipEventCreate(&x));
+      tev.push_back(x);
+    }
+    *e = tev[tev_used++];
+    return PA_OK;
+  }
+
+  // Adds the recorded per-step kernel times to sweep_ms / resample_ms; call after a sync.
+  int collect_timing() {
+    for (size_t i = 0; i + 2 < tev_used; i += 3) {
+      float a = 0, b = 0;
+      HIPCHECK(hipEventElapsedTime(&a, tev[i], tev[i + 1]));
+      HIPCHECK(hipEventElapsedTime(&b, tev[i + 1], tev[i + 2]));
+      resample_ms += a;
+      sweep_ms += b;
+      ++timed_steps;
+    }
+    tev_used = 0;
+    return PA_OK;
+  }
+
   int launch_step(bool time_sweep) {
     const int n = (int)runs.size();
+    hipEvent_t t0 = nullptr, t1 = nullptr, t2 = nullptr;
+    if (time_sweep) {
+      int rc;
+      if ((rc = timing_event(&t0)) || (rc = timing_event(&t1)) || (rc = timing_event(&t2))) return rc;
+    }
     if (has_w[0] || has_w[1])
       HIPCHECK(pa_launch_step(8, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(1, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(2, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(9, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
+    if (time_sweep) HIPCHECK(hipEventRecord(t0, stream));
     if (has_w[0] || has_w[1] ? has_ssc() : true)
       HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, blocks[kKResample], 0, stream, desc_d, offs_d, n));
     for (int wi = 0; wi < 2; ++wi)
       if (has_w[wi])
         HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, 0, blocks[kKResample], rs_lds[wi], stream,
                                 desc_d, offs_d, n));
-    if (time_sweep) HIPCHECK(hipEventRecord(evs0, stream));
+    if (time_sweep) HIPCHECK(hipEventRecord(t1, stream));
     for (const Group& g : groups) {
       const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;
       if (g.bal_workers > 0)
@@ -519,7 +565,7 @@ struct Batch {
         HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.units_blocks * ns, g.lds, stream,
                                 g.desc_d, g.offs_d, (int)g.members.size()));
     }
-    if (time_sweep) HIPCHECK(hipEventRecord(evs1, stream));
+    if (time_sweep) HIPCHECK(hipEventRecord(t2, stream));
     HIPCHECK(pa_launch_step(5, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n));
     if (any_trace) HIPCHECK(pa_launch_step(6, 0, 0, 0, 0, blocks[kKUnits], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(7, 0, 0, 0, 0, n, 0, stream, desc_d, offs_d, n));
@@ -548,10 +594,6 @@ struct Batch {
       } else if (timing) {
         int rc = launch_step(true);
         if (rc) return rc;
-        HIPCHECK(hipEventSynchronize(evs1));
-        float ms = 0;
-        HIPCHECK(hipEventElapsedTime(&ms, evs0, evs1));
-        sweep_ms += ms;
       } else {
         int rc = launch_step(false);
         if (rc) return rc;
@@ -570,8 +612,6 @@ struct Batch {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&ev0));
     HIPCHECK(hipEventCreate(&ev1));
-    HIPCHECK(hipEventCreate(&evs0));
-    HIPCHECK(hipEventCreate(&evs1));
     return PA_OK;
   }
 };
@@ -739,6 +779,8 @@ int pa_run_sync(pa_run* r) {
   float ms = 0;
   HIPCHECK(hipEventElapsedTime(&ms, r->batch.ev0, r->batch.ev1));
   r->batch.total_ms = ms;
+  int rc = r->batch.collect_timing();
+  if (rc) return rc;
   return fetch(*r->host, r->batch.stream);
 }
 
@@ -821,6 +863,14 @@ int pa_run_timing(pa_run* r, double* sweep_ms, double* total_ms) {
   if (!r) return fail(PA_ERR_INVALID, "null run");
   if (sweep_ms) *sweep_ms = r->batch.sweep_ms;
   if (total_ms) *total_ms = r->batch.total_ms;
+  return PA_OK;
+}
+
+int pa_run_kernel_times(pa_run* r, double* resample_ms, double* sweep_ms, int* steps) {
+  if (!r) return fail(PA_ERR_INVALID, "null run");
+  if (resample_ms) *resample_ms = r->batch.resample_ms;
+  if (sweep_ms) *sweep_ms = r->batch.sweep_ms;
+  if (steps) *steps = r->batch.timed_steps;
   return PA_OK;
 }
 

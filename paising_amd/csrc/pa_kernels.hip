@@ -149,21 +149,58 @@ __device__ __forceinline__ S msc_acc(S s, S n1, S n2, S n3, S n4, S c4, S c8) {
   return s12 | (~s12 & ((sum0 & c4) | (~sum0 & c8)));
 }
 
-// Condition masks for SW consecutive LCG values starting at `ran` (4 independent chains
-// interleaved for instruction-level parallelism).
+// One LCG step, ran' = A ran + C (mod 2^32), as a single v_mad_u64_u32 (A in an SGPR, C as
+// a 64-bit VGPR pair); the compiler would otherwise emit v_mul_lo_u32 + v_add_u32.
+__device__ __forceinline__ uint32_t lcg_next(uint32_t r, uint64_t c64) {
+  uint64_t p, carry;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(p), "=s"(carry) : "s"(kLcgA), "v"(r), "v"(c64));
+  return (uint32_t)p;
+}
+
+// a4 = 2 a4 + [r < T4], a8 = 2 a8 + [r < T8]: a compare into an SGPR lane mask and an add
+// with carry-in per mask (the compiler's form is compare + select + or).  The s_nop keeps
+// two wait states between each mask write and its read as carry-in.
+__device__ __forceinline__ void lcg_cmp2(uint32_t& a4, uint32_t& a8, uint32_t r, uint32_t T4,
+                                         uint32_t T8) {
+  uint64_t m4, m8, c0, c1;
+  asm("v_cmp_gt_u32_e64 %[m4], %[t4], %[r]\n\t"
+      "v_cmp_gt_u32_e64 %[m8], %[t8], %[r]\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32_e64 %[a4], %[c0], %[a4], %[a4], %[m4]\n\t"
+      "v_addc_co_u32_e64 %[a8], %[c1], %[a8], %[a8], %[m8]"
+      : [a4] "+v"(a4), [a8] "+v"(a8), [m4] "=&s"(m4), [m8] "=&s"(m8), [c0] "=&s"(c0),
+        [c1] "=&s"(c1)
+      : [t4] "s"(T4), [t8] "s"(T8), [r] "v"(r));
+}
+
+// Condition masks for SW consecutive LCG values starting at `ran` (NW independent chains
+// interleaved for instruction-level parallelism): bit i of c4 (c8) is [ran_i < T4]
+// ([ran_i < T8]).  The masks are accumulated most-significant-first and bit-reversed.
 template <typename S, int NW>
 __device__ __forceinline__ void lcg_masks(uint32_t (&ran)[NW], uint32_t T4, uint32_t T8,
                                           S (&c4)[NW], S (&c8)[NW]) {
   constexpr int SW = (int)sizeof(S) * 8;
+  constexpr int HB = SW < 32 ? SW : 32;  // bits per 32-bit accumulator
+  uint64_t c64 = kLcgC;
 #pragma unroll
   for (int j = 0; j < NW; ++j) { c4[j] = 0; c8[j] = 0; }
 #pragma unroll
-  for (int i = 0; i < SW; ++i) {
+  for (int half = 0; half < SW / HB; ++half) {
+    uint32_t a4[NW], a8[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) { a4[j] = 0; a8[j] = 0; }
+#pragma unroll
+    for (int i = 0; i < HB; ++i) {
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        lcg_cmp2(a4[j], a8[j], ran[j], T4, T8);
+        ran[j] = lcg_next(ran[j], c64);
+      }
+    }
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
-      c4[j] |= (S)(ran[j] < T4) << i;
-      c8[j] |= (S)(ran[j] < T8) << i;
-      ran[j] = kLcgA * ran[j] + kLcgC;
+      c4[j] |= (S)((uint64_t)(__builtin_bitreverse32(a4[j]) >> (32 - HB)) << (32 * half));
+      c8[j] |= (S)((uint64_t)(__builtin_bitreverse32(a8[j]) >> (32 - HB)) << (32 * half));
     }
   }
 }
@@ -544,8 +581,13 @@ __global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict_
   RunState* st = d.st;
   if (!st->active || st->overflow) return;
   if (d.coding != kMSC || d.w != w) return;
+  // The run's blocks come in sets of 8 * rs_lp * rs_split starting at a multiple of 8.  The
+  // blocks of the rs_lp chunks of one 128-byte line (and their rs_split splits) are 8 apart,
+  // so they share an XCD and the line is fetched into that XCD's L2 once.
   const int cb = blockIdx.x - off[r];
-  const int c = cb / d.rs_split, js = cb % d.rs_split;
+  const int per = 8 * d.rs_lp * d.rs_split;
+  const int q = cb / per, rem = cb - q * per, t = rem >> 3;
+  const int c = (q * 8 + (rem & 7)) * d.rs_lp + t % d.rs_lp, js = t / d.rs_lp;
   const int S = d.rs_sites, N = d.N;
   const int s0 = c * S;
   if (s0 >= N) return;
@@ -569,9 +611,10 @@ __global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict_
       const int64_t n = (int64_t)k * rgn + j;
       pl[k] = (n < Rn) ? d.par[n] : -1;
     }
-    W acc[16];
+    constexpr int SM = 64 / (int)sizeof(W);  // largest chunk: 64 bytes per group
+    W acc[SM];
 #pragma unroll
-    for (int s = 0; s < 16; ++s) acc[s] = 0;
+    for (int s = 0; s < SM; ++s) acc[s] = 0;
 #pragma unroll
     for (int k = 0; k < w; ++k) {
       const int p = pl[k];
@@ -579,12 +622,12 @@ __global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict_
       const int pg = p / w, pb = p - pg * w;
       const W* o = old + pg * S;
 #pragma unroll
-      for (int s = 0; s < 16; ++s)
+      for (int s = 0; s < SM; ++s)
         if (s < ns) acc[s] |= ((o[s] >> pb) & (W)1) << k;
     }
     W* out = dst + (size_t)j * N + s0;
 #pragma unroll
-    for (int s = 0; s < 16; ++s)
+    for (int s = 0; s < SM; ++s)
       if (s < ns) out[s] = acc[s];
     if (c == 0) {
       uint64_t m = 0;

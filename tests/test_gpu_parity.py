@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     # (L, coding, msbits, Rinit, eqsweeps, dB, Bfin, seed, steps)
-    (16, pa.SSC, 32, 1000, 10, 0.005, 1.0, 100, 8),      # configs[0] shape, truncated
+    (16, pa.SSC, 32, 1000, 10, 0.005, 1.0, 100, 8),      # L=16 R=1000 EqSw=10 shape, truncated
     (32, pa.SSC, 32, 500, 5, 0.01, 1.0, 7, 6),
     (24, pa.SSC, 32, 300, 3, 0.02, 1.0, 11, 5),           # N/2 not a multiple of 128
     (16, pa.MSC, 32, 1000, 10, 0.005, 1.0, 100, 8),
@@ -62,3 +62,23 @@ def test_balanced_batch_of_mixed_sizes():
         assert [r.spin_hash for r in plain] == [r.spin_hash for r in got]
         assert [r.lnQ.hex() for r in plain] == [r.lnQ.hex() for r in got]
     b.close()
+
+
+@pytest.mark.parametrize("L,w,sw,nt", [(32, 32, 16, 128), (32, 32, 8, 256), (64, 32, 16, 512),
+                                       (32, 64, 32, 256), (32, 64, 16, 512), (48, 32, 8, 128),
+                                       (64, 32, 32, 128), (64, 32, 32, 256)])
+def test_slices_and_block_shapes_match_oracle(L, w, sw, nt):
+    """Every replica-bit slice width and block shape the autotuner may pick is bit-exact."""
+    args = (L, pa.MSC, w, 700, 4, 0.01, 1.0, 13, 4)
+    print(f"case L={L} w={w} sw={sw} nt={nt}", flush=True)
+    orows = oracle_rows(*args)
+    erows, _ = pa.run_all(engine_params(*args, slice_bits=sw, threads=nt))
+    assert compare_rows(orows, erows) == 5
+
+
+@pytest.mark.parametrize("nt", [128, 256, 512])
+def test_ssc_block_shapes_match_oracle(nt):
+    args = (64, pa.SSC, 32, 300, 3, 0.01, 1.0, 17, 3)
+    orows = oracle_rows(*args)
+    erows, _ = pa.run_all(engine_params(*args, threads=nt))
+    assert compare_rows(orows, erows) == 4
