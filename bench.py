@@ -48,8 +48,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload (FETCH_SIZE scaled x2, WRITE_SIZE x1, the factors a plain device copy of known
-# size reads back); summaries in profiles/.
-TRAFFIC_BYTES = {"sweep_msc32": None, "resample_msc32": None}
+# size reads back): profiles/k_sweep_msc_bal_msc32_L64_R20000.md, k_resample_L64_R20000.md.
+TRAFFIC_BYTES = {"sweep_msc32": 37.56e6, "resample_msc32": 21.57e6}
 
 
 def msc_ops_per_attempt(w=32):
@@ -253,7 +253,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "copy", "config3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=6, help="beta steps in the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=24, help="beta steps in the CPU baseline sample")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the planner's default sweep variant instead of autotuning")
     ap.add_argument("--no-ssc", action="store_true", help="skip the SSC line")
     ap.add_argument("--nruns", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=0, help="truncate batched runs (0 = full)")
@@ -282,7 +284,7 @@ def main():
     name, cu, _ = pa.device_info()
     clock = device_clock_hz()
     peak = peak_lane_ops_per_s(cu, clock)
-    m = timed_run(pa.MSC, args.steps, args.warmup)
+    m = timed_run(pa.MSC, args.steps, args.warmup, tune=not args.no_tune)
     value = m["attempts"] / (m["wall"] * 1e9)
     ops = msc_ops_per_attempt(32)
     achieved = m["attempts"] * ops / (m["sweep_ms"] * 1e-3)
@@ -317,7 +319,7 @@ def main():
                               "resample_ms_per_step": round(m["resample_ms"] / args.steps, 4)},
     }
     if not args.no_ssc:
-        s_ = timed_run(pa.SSC, args.steps, args.warmup)
+        s_ = timed_run(pa.SSC, args.steps, args.warmup, tune=not args.no_tune)
         s_ach = s_["attempts"] * SSC_OPS_PER_SITE / (s_["sweep_ms"] * 1e-3)
         out["ssc"] = {"value": round(s_["attempts"] / (s_["wall"] * 1e9), 4),
                       "unit": "spin-flip attempts/ns",

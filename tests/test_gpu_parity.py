@@ -82,3 +82,15 @@ def test_ssc_block_shapes_match_oracle(nt):
     orows = oracle_rows(*args)
     erows, _ = pa.run_all(engine_params(*args, threads=nt))
     assert compare_rows(orows, erows) == 4
+
+
+@pytest.mark.parametrize("L,nt", [(96, 0), (96, 256), (48, 0), (16, 0), (10, 0), (6, 0), (4, 0),
+                                  (130, 0), (160, 0), (64, 128)])
+def test_ssc_lattices_off_the_fast_path_match_oracle(L, nt):
+    """SSC lattices whose half sweeps do not split into whole Philox blocks per thread
+    (stream walk), or need several blocks per thread, at every block shape."""
+    args = (L, pa.SSC, 32, 120, 3, 0.02, 1.0, 29, 3)
+    print(f"case L={L} nt={nt}", flush=True)
+    orows = oracle_rows(*args)
+    erows, _ = pa.run_all(engine_params(*args, threads=nt))
+    assert compare_rows(orows, erows) == 4
