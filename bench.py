@@ -260,7 +260,7 @@ def main():
     ap.add_argument("--nruns", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=0, help="truncate batched runs (0 = full)")
     ap.add_argument("--nreq", type=int, default=16)
-    ap.add_argument("--tick-steps", type=int, default=8)
+    ap.add_argument("--tick-steps", type=int, default=32)
     ap.add_argument("--check", type=int, default=4, help="served runs re-run on the plain path")
     args = ap.parse_args()
     if args.gpus != 1:

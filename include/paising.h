@@ -148,7 +148,9 @@ int pa_server_tick(pa_server* s, int* completed);
 int pa_server_poll(pa_server* s, int64_t ticket, int* done);
 int pa_server_results(pa_server* s, int64_t ticket, pa_row* rows, int max_rows, int* nrows);
 int pa_server_stats(pa_server* s, int* queued, int* resident, size_t* hbm_used);
-/* Steps advanced per tick and whether ticks replay a captured hipGraph. */
+/* Scheduling of a tick: every resident run gets about the same device time, i.e. a step
+ * quota inversely proportional to its work per step (EqSw * N * Rinit), at least 1 and at
+ * most steps_per_tick (default 32); use_graph: ticks replay a captured hipGraph. */
 int pa_server_set_chunk(pa_server* s, int steps_per_tick, int use_graph);
 /* Seconds from submission (and from admission) to completion of a finished request. */
 int pa_server_latency(pa_server* s, int64_t ticket, double* submit_to_done_s,

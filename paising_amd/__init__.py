@@ -287,7 +287,7 @@ class Batch:
 class Server:
     """Continuous-batching server: submit runs, tick, poll, fetch results."""
 
-    def __init__(self, hbm_budget_bytes, max_runs=256, steps_per_tick=8, use_graph=True):
+    def __init__(self, hbm_budget_bytes, max_runs=256, steps_per_tick=32, use_graph=True):
         h = ctypes.c_void_p()
         _check(lib().pa_server_create(hbm_budget_bytes, max_runs, ctypes.byref(h)))
         self.h = h

@@ -28,7 +28,7 @@ struct RunState {
   int32_t overflow;  // set when R_i exceeds the buffer capacity; the run then stops
   int32_t active;    // 0 once the run has finished its schedule
   int32_t err;       // device-side error bits (1: balanced-sweep hand-over wait timed out)
-  int32_t pad;
+  int32_t pause;     // serving: the run stops (active = 0) after this step; 0 = never
 };
 
 // Per-step record written by the finalize kernel (row i of the results file + internals).

@@ -34,6 +34,8 @@ extern "C" hipError_t pa_prepare_kernels(void);
 extern "C" hipError_t pa_occupancy(int kind, int coding, int w, int sw, int nt, size_t lds,
                                    int* blocks_per_cu);
 extern "C" hipError_t pa_launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+extern "C" hipError_t pa_launch_resume(hipStream_t s, const pa::RunDesc* runs, int nruns,
+                                       const int32_t* pause);
 
 namespace {
 
@@ -366,6 +368,7 @@ struct Batch {
   RunDesc* desc_d = nullptr;
   int32_t* offs_d = nullptr;
   uint64_t* is0_d = nullptr;
+  int32_t* pause_d = nullptr;  // serving: last step of the tick per run
   int blocks[kKindCount] = {0, 0, 0, 0, 0};
   size_t rs_lds[2] = {0, 0};  // MSC resampling LDS for w = 32, 64
   bool has_w[2] = {false, false};
@@ -446,6 +449,7 @@ struct Batch {
     if ((rc = alloc_table((void**)&desc_d, n * sizeof(RunDesc)))) return rc;
     if ((rc = alloc_table((void**)&offs_d, offs.size() * 4))) return rc;
     if ((rc = alloc_table((void**)&is0_d, n * 8))) return rc;
+    if ((rc = alloc_table((void**)&pause_d, n * 4))) return rc;
     HIPCHECK(hipMemcpy(desc_d, descs.data(), n * sizeof(RunDesc), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(offs_d, offs.data(), offs.size() * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(is0_d, is0.data(), n * 8, hipMemcpyHostToDevice));
@@ -1043,6 +1047,7 @@ struct Request {
   std::unique_ptr<RunHost> h;
   int state = 0;  // 0 queued, 1 resident, 2 done, 3 failed
   int steps_left = 0;
+  double credit = 0;  // serving: step credit carried between ticks
   int rc = PA_OK;
   std::vector<pa_row> rows;
   double t_submit = 0, t_admit = 0, t_done = 0;
@@ -1064,8 +1069,9 @@ struct pa_server {
   std::vector<int64_t> resident;
   int64_t next_ticket = 1;
   int completed = 0;
-  int chunk = 8;
+  int chunk = 32;
   bool dirty = false;
+  std::vector<int32_t> pause, quota;
 };
 
 extern "C" {
@@ -1179,10 +1185,35 @@ int pa_server_tick(pa_server* s, int* completed) {
     s->dirty = false;
   }
   if (!s->resident.empty()) {
-    int k = s->chunk;
-    for (int64_t t : s->resident) k = std::min(k, s->reqs[t].steps_left);
-    if ((rc = s->batch.run_steps(k))) return rc;
-    for (int64_t t : s->resident) s->reqs[t].steps_left -= k;
+    // Equal device time per resident run (processor sharing): run i earns c_max / c_i
+    // steps of credit per tick, c = EqSw * N * Rinit its work per step, and runs the whole
+    // part of its credit (at most `chunk`); it pauses on the device after its quota while
+    // the tick replays the step max_i k_i times.  Cheap runs therefore finish in few ticks
+    // instead of advancing in lockstep with the expensive ones.
+    const size_t n = s->resident.size();
+    auto cost = [&](int64_t t) {
+      const RunHost& h = *s->reqs[t].h;
+      return (double)h.p.eqsweeps * h.N * h.R0 + 1.0;
+    };
+    double cmax = 0;
+    for (int64_t t : s->resident) cmax = std::max(cmax, cost(t));
+    s->pause.resize(n);
+    s->quota.resize(n);
+    int K = 0;
+    for (size_t i = 0; i < n; ++i) {
+      Request& rq = s->reqs[s->resident[i]];
+      rq.credit = std::min(rq.credit + cmax / cost(rq.ticket), (double)s->chunk);
+      const int k = std::min((int)rq.credit, rq.steps_left);
+      rq.credit -= k;
+      s->quota[i] = k;
+      s->pause[i] = rq.h->n_steps - rq.steps_left + k;  // k = 0: paused for this tick
+      K = std::max(K, k);
+    }
+    HIPCHECK(hipMemcpyAsync(s->batch.pause_d, s->pause.data(), n * sizeof(int32_t),
+                            hipMemcpyHostToDevice, s->batch.stream));
+    HIPCHECK(pa_launch_resume(s->batch.stream, s->batch.desc_d, (int)n, s->batch.pause_d));
+    if ((rc = s->batch.run_steps(K))) return rc;
+    for (size_t i = 0; i < n; ++i) s->reqs[s->resident[i]].steps_left -= s->quota[i];
     if ((rc = server_retire(s))) return rc;
   }
   if (completed) *completed = s->completed;

@@ -1100,7 +1100,17 @@ __global__ void k_fin(const RunDesc* __restrict__ runs, int nruns) {
   st->R = Rn;
   st->buf ^= 1;
   st->step = step + 1;
-  if (step + 1 > d.n_steps) st->active = 0;
+  if (step + 1 > d.n_steps || step == st->pause) st->active = 0;
+}
+
+// Serving: re-activate the resident runs for a tick; run r executes steps up to pause[r].
+__global__ void k_resume(const RunDesc* __restrict__ runs, int nruns,
+                         const int32_t* __restrict__ pause) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  RunState* st = runs[r].st;
+  st->pause = pause[r];
+  st->active = (!st->overflow && st->step <= runs[r].n_steps && st->step <= pause[r]) ? 1 : 0;
 }
 
 }  // namespace pa
@@ -1184,6 +1194,12 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 extern "C" {
 
 using namespace pa;
+
+hipError_t pa_launch_resume(hipStream_t s, const RunDesc* runs, int nruns, const int32_t* pause) {
+  if (nruns <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_resume, dim3((nruns + 255) / 256), dim3(256), 0, s, runs, nruns, pause);
+  return hipGetLastError();
+}
 
 hipError_t pa_launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
   const size_t n = bytes / 16;

@@ -73,6 +73,25 @@ def test_server_matches_plain_alone_and_in_mix():
     srv.close()
 
 
+def test_server_shares_device_time():
+    """A cheap request submitted after an expensive one finishes first (each resident run
+    gets about the same device time per tick), and both stay bit-exact."""
+    big = pa.make_params(L=64, coding=pa.MSC, msbits=32, Rinit=20000, eqsweeps=50, dB=0.005,
+                         Bfin=1.0, seed=71)
+    small = pa.make_params(L=16, coding=pa.SSC, Rinit=500, eqsweeps=5, dB=0.005, Bfin=1.0, seed=72)
+    srv = pa.Server(hbm_budget_bytes=1 << 30, max_runs=8, steps_per_tick=32)
+    tb = srv.submit(big)
+    ts = srv.submit(small)
+    while not srv.done(ts):
+        srv.tick()
+    assert not srv.done(tb)
+    while not srv.done(tb):
+        srv.tick()
+    same(pa.run_all(big)[0], srv.results(tb))
+    same(pa.run_all(small)[0], srv.results(ts))
+    srv.close()
+
+
 def test_server_respects_budget():
     p = engine_params(64, pa.MSC, 32, 20000, 1, 0.005, 1.0, 1, 2)
     need = pa.footprint(p)
