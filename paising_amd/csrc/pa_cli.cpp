@@ -156,6 +156,17 @@ int main(int argc, char** argv) {
             << " replicas: " << (double)fp / 1024.0 / 1024.0 << " MB \n";
   fflush(stdout);
 
+  // The reference creates its device context (cudaDeviceSetCacheConfig, cudaEventCreate)
+  // before its timer starts (PAisingSSC.cu:439-445); initialise the device the same way.
+  {
+    char name[128];
+    int cus = 0;
+    size_t hbm = 0;
+    if (pa_device_info(name, sizeof name, &cus, &hbm) != PA_OK) {
+      fprintf(stderr, "%s\n", pa_last_error());
+      return 1;
+    }
+  }
   const auto t0 = std::chrono::steady_clock::now();
   double totPop = 0.0;
   unsigned long long IS = 0;
