@@ -80,20 +80,38 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(sample_steps, threads):
-    """The oracle (host C++/OpenMP restatement of the reference kernels) on a bounded sample
-    of the same workload: initial population plus the first `sample_steps` beta steps of
-    the L=64 R=20000 MSC32 run, timed on this host."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_rate(coding, sample_steps, threads):
+    """The oracle (host C++/OpenMP restatement of the reference kernels) on the initial
+    population plus the first `sample_steps` beta steps of the L=64 R=20000 run."""
     import oracle
-    rc, runs, el, tot = oracle.run(L=L, coding=oracle.MSC, msbits=32, Rinit=RINIT, eqsweeps=EQSW,
+    rc, runs, el, tot = oracle.run(L=L, coding=coding, msbits=32, Rinit=RINIT, eqsweeps=EQSW,
                                    dB=DB, Bfin=1.0, seed=SEED, max_steps=sample_steps,
                                    threads=threads)
     assert rc == 0
-    att = EQSW * N * tot
-    return {"value": round(att / (el * 1e9), 4), "unit": "spin-flip attempts/ns", "cores": threads,
+    return EQSW * N * tot / (el * 1e9), el
+
+
+def cpu_baseline(sample_steps, threads):
+    """Bounded CPU sample of the headline workload (MSC32), on all host threads and on one."""
+    import oracle
+    v, el = oracle_rate(oracle.MSC, sample_steps, threads)
+    v1, el1 = oracle_rate(oracle.MSC, 1, 1)
+    return {"value": round(v, 4), "unit": "spin-flip attempts/ns", "cores": threads,
             "kind": "port",
             "sample": f"oracle (C++/OpenMP), init + first {sample_steps} beta steps of L=64 R=20000 "
-                      f"EqSw=100 MSC32, {el:.1f} s wall on {threads} threads"}
+                      f"EqSw=100 MSC32, {el:.1f} s wall on {threads} threads",
+            "single_thread": {"value": round(v1, 4), "sample": f"init + 1 beta step, {el1:.1f} s"},
+            "cpu": cpu_model()}
 
 
 def timed_run(coding, steps, warmup, tune=True):
@@ -331,6 +349,12 @@ def main():
                       "variant": s_["variant"]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, cpu_threads())
+        if "ssc" in out:
+            import oracle
+            v, el = oracle_rate(oracle.SSC, 1, cpu_threads())
+            out["ssc"]["cpu_baseline"] = {
+                "value": round(v, 4), "unit": "spin-flip attempts/ns", "cores": cpu_threads(),
+                "kind": "port", "sample": f"oracle, init + 1 beta step of the SSC run, {el:.1f} s"}
     print(json.dumps(out), flush=True)
 
 
