@@ -58,6 +58,13 @@ typedef struct pa_params {
   int32_t slice_bits;        /* MSC sweep slice width 8/16/32/64, 0 = autotuned           */
   int32_t threads;           /* sweep workgroup size 128/256/512, 0 = autotuned           */
   int32_t balance;           /* MSC makespan-balanced sweeps: 0 auto, 1 on, -1 off        */
+  int32_t adaptive;          /* 1 = adaptive beta step, the reference's AdaptiveStep build  */
+                             /* (PAisingSSC.cu:23,644-660): plain runs only               */
+  int32_t schedule_len;      /* > 0: run the explicit schedule B[0..len-1] instead of the  */
+                             /* dB ladder (replay of run 0's steps, PAisingSSC.cu:662)    */
+  double min_overlap;        /* adaptive: MinOverlap, 0 = 0.85 (PAisingSSC.cu:41)          */
+  double max_overlap;        /* adaptive: MaxOverlap, 0 = 0.87 (PAisingSSC.cu:42)          */
+  const double* schedule;    /* explicit schedule (B[0] = Binit), copied at creation      */
 } pa_params;
 
 /* One row of PA_results_run_<r>.dat (PAisingSSC.cu:679-690) and the step internals. */

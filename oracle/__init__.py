@@ -29,6 +29,8 @@ class Params(ctypes.Structure):
         ("Binit", ctypes.c_double), ("Bfin", ctypes.c_double), ("runs", ctypes.c_int32),
         ("seed", ctypes.c_uint64), ("initial_sequence", ctypes.c_uint64),
         ("umap", ctypes.c_int32), ("max_steps", ctypes.c_int32), ("threads", ctypes.c_int32),
+        ("adaptive", ctypes.c_int32), ("pad0", ctypes.c_int32), ("min_overlap", ctypes.c_double),
+        ("max_overlap", ctypes.c_double),
     ]
 
 
@@ -80,6 +82,8 @@ def lib():
             "ora_ssc_q": (D, [I, P, D, D]),
             "ora_msc_q": (D, [I, I, P, P, D, D]),
             "ora_lnq": (D, [D, D, D, I]),
+            "ora_ssc_overlap": (D, [I, P, I, D, D]),
+            "ora_msc_overlap": (D, [I, I, P, P, I, I, D, D]),
             "ora_ssc_averages": (None, [I, P, P, P]),
             "ora_msc_averages": (None, [I, I, P, P, P, P]),
             "ora_ssc_resample": (None, [I, I, P, P, P]),
@@ -101,10 +105,11 @@ def _ptr(a):
 
 def run(L=64, coding=SSC, msbits=32, Rinit=20000, eqsweeps=100, dB=0.005, Binit=0.0,
         Bfin=1.0, runs=1, seed=100, initial_sequence=0, umap=U_DEFAULT, max_steps=0,
-        threads=0, max_rows=10000):
+        threads=0, max_rows=10000, adaptive=0, min_overlap=0.0, max_overlap=0.0):
     """Run the oracle; returns (rc, list of per-run row lists, elapsed_s, total_pop)."""
     p = Params(L, coding, msbits, Rinit, eqsweeps, dB, Binit, Bfin, runs,
-               seed & (2**64 - 1), initial_sequence, umap, max_steps, threads)
+               seed & (2**64 - 1), initial_sequence, umap, max_steps, threads, adaptive, 0,
+               min_overlap, max_overlap)
     rows = (Row * (max_rows * runs))()
     nrows = np.zeros(runs, dtype=np.int32)
     el = ctypes.c_double(0)

@@ -667,6 +667,82 @@ void set_threads(int n) {
 #endif
 }
 
+/* HistogramOverlap + CalcOverlap, SSC:308-334: q and lnQ of the candidate step as in the
+ * step itself (QKer, SSC:533), then sum over replicas of min(1, Rinit/R exp(-dB E - lnQ))
+ * with the block tree; the reference adds the block sums with atomicAdd (order unspecified),
+ * here they are added in launch order.  Returns the sum divided by R. */
+double ssc_overlap(int R, const int32_t* IE, int Rinit, double dB, double Emean) {
+  const double q = ssc_q(R, IE, dB, Emean);
+  const double lnQ = lnq(dB, Emean, q, R);
+  const double ratio = (double)Rinit / R;
+  int nb = (R + NT - 1) / NT;
+  std::vector<double> v(NT);
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    for (int t = 0; t < NT; ++t) {
+      int idx = t + NT * b;
+      v[t] = (idx < R) ? std::min(1.0, ratio * my_exp(std::fma(-dB, (double)IE[idx], -lnQ))) : 0.0;
+    }
+    acc += block_reduce(v.data(), NT);
+  }
+  return acc / R;
+}
+
+/* MSC:466-499: the same over active lanes; grid (blocks of 1024 groups) x (w lanes), block
+ * sums added in launch order (lane-major over blocks), as for the averages. */
+double msc_overlap(int w, int rg, const uint8_t* active, const int32_t* IE, int Rinit, int R,
+                   double dB, double Emean) {
+  const double q = msc_q(w, rg, active, IE, dB, Emean);
+  const double lnQ = lnq(dB, Emean, q, R);
+  const double ratio = (double)Rinit / R;
+  int nb = (rg + NT - 1) / NT;
+  std::vector<double> v(NT);
+  double acc = 0.0;
+  for (int by = 0; by < w; ++by) {
+    for (int b = 0; b < nb; ++b) {
+      for (int t = 0; t < NT; ++t) {
+        int idx = t + NT * b;
+        size_t k = (size_t)idx * w + by;
+        v[t] = (idx < rg && active[k])
+                   ? std::min(1.0, ratio * my_exp(std::fma(-dB, (double)IE[k], -lnQ)))
+                   : 0.0;
+      }
+      acc += block_reduce(v.data(), NT);
+    }
+  }
+  return acc / R;
+}
+
+/* Next inverse-temperature step of the adaptive schedule, SSC:644-660 / MSC:795-811: grow
+ * the step by 10 % while the overlap stays >= MaxOverlap (and B + step < Bfin), then bisect
+ * on [0, step] until MinOverlap <= overlap < MaxOverlap; the last step lands on Bfin.
+ * `ovl(dB)` evaluates CalcOverlap on the current population. */
+template <typename F>
+int adaptive_step(const ora_params* p, double Bprev, double& deltaBeta, F ovl) {
+  double overlap, dBmin = 0, dBmax = deltaBeta, dBmean;
+  for (int it = 0;; ++it) {
+    if (it > 100000) return -1;
+    overlap = ovl(dBmax);
+    if ((overlap >= p->max_overlap) && (Bprev + dBmax < p->Bfin)) dBmax *= 1.1;
+    else break;
+  }
+  if (overlap >= p->min_overlap) {
+    dBmean = dBmax;
+  } else {
+    for (int it = 0;; ++it) {
+      if (it > 100000) return -1;
+      dBmean = 0.5 * (dBmin + dBmax);
+      overlap = ovl(dBmean);
+      if (overlap < p->min_overlap) dBmax = dBmean;
+      else if (overlap >= p->max_overlap) dBmin = dBmean;
+      else break;
+    }
+  }
+  if ((Bprev < p->Bfin) && (Bprev + dBmean > p->Bfin)) deltaBeta = p->Bfin - Bprev;
+  else deltaBeta = dBmean;
+  return 0;
+}
+
 /* ---------------- driver: main() of SSC:381-806 / MSC:546-957 ---------------- */
 struct Thermo {
   std::vector<double> B, Binc, E, C, M, M2, M4, lnQ, BF, S;
@@ -773,8 +849,17 @@ int run_msc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
         fprintf(stderr, "Error: number of temperature steps exceeds nBmax=%d.\n", NBMAX);
         return 1;
       }
-      if (r == 0) th.B[i] = th.Binc[i] = th.B[iprev] + deltaBeta;
-      else th.B[i] = th.Binc[i];
+      if (r == 0) {
+        if (p->adaptive &&
+            adaptive_step(p, th.B[iprev], deltaBeta, [&](double dBc) {
+              return msc_overlap(w, rg, active.data(), IE.data(), p->Rinit, th.R[iprev], dBc,
+                                 Emean);
+            }))
+          return -1;
+        th.B[i] = th.Binc[i] = th.B[iprev] + deltaBeta;
+      } else {
+        th.B[i] = th.Binc[i];
+      }
     }
     int nB = i;
     nrows[r] = nB < max_rows ? nB : max_rows;
@@ -871,8 +956,16 @@ int run_ssc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
         fprintf(stderr, "Error: number of temperature steps exceeds nBmax=%d.\n", NBMAX);
         return 1;
       }
-      if (r == 0) th.B[i] = th.Binc[i] = th.B[iprev] + deltaBeta;
-      else th.B[i] = th.Binc[i];
+      if (r == 0) {
+        if (p->adaptive &&
+            adaptive_step(p, th.B[iprev], deltaBeta, [&](double dBc) {
+              return ssc_overlap(th.R[iprev], IE.data(), p->Rinit, dBc, Emean);
+            }))
+          return -1;
+        th.B[i] = th.Binc[i] = th.B[iprev] + deltaBeta;
+      } else {
+        th.B[i] = th.Binc[i];
+      }
     }
     int nB = i;
     nrows[r] = nB < max_rows ? nB : max_rows;
@@ -902,6 +995,12 @@ int ora_run(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
             double* total_pop) {
   if (!p || p->L < 2 || (p->L & 1) || p->Rinit < 1 || p->runs < 1) return -1;
   if (p->coding == ORA_MSC && p->msbits != 32 && p->msbits != 64) return -1;
+  ora_params pp = *p;
+  if (pp.adaptive && pp.min_overlap == 0.0 && pp.max_overlap == 0.0) {
+    pp.min_overlap = 0.85;  // SSC:41
+    pp.max_overlap = 0.87;  // SSC:42
+  }
+  p = &pp;
   set_threads(p->threads);
   memset(rows, 0, sizeof(ora_row) * (size_t)max_rows * p->runs);
   auto t0 = std::chrono::steady_clock::now();
@@ -983,6 +1082,13 @@ double ora_msc_q(int w, int rg, const uint8_t* active, const int32_t* IE, double
   return msc_q(w, rg, active, IE, dB, Emean);
 }
 double ora_lnq(double dB, double Emean, double q, int Rprev) { return lnq(dB, Emean, q, Rprev); }
+double ora_ssc_overlap(int R, const int32_t* IE, int Rinit, double dB, double Emean) {
+  return ssc_overlap(R, IE, Rinit, dB, Emean);
+}
+double ora_msc_overlap(int w, int rg, const uint8_t* active, const int32_t* IE, int Rinit,
+                       int R, double dB, double Emean) {
+  return msc_overlap(w, rg, active, IE, Rinit, R, dB, Emean);
+}
 void ora_ssc_averages(int R, const int32_t* IE, const int32_t* M, double out[5]) {
   ssc_averages(R, IE, M, out);
 }

@@ -50,6 +50,10 @@ typedef struct {
   int32_t umap;         /* ORA_U_* mapping                                    */
   int32_t max_steps;    /* stop after this many beta steps (0 = full schedule) */
   int32_t threads;      /* OpenMP threads (0 = runtime default)               */
+  int32_t adaptive;     /* 1 = adaptive beta step (#define AdaptiveStep, SSC:23) */
+  int32_t pad0;
+  double min_overlap;   /* MinOverlap (SSC:41)                                */
+  double max_overlap;   /* MaxOverlap (SSC:42)                                */
 } ora_params;
 
 /* One row of PA_results_run_r.dat (SSC:679-690) plus the per-step internals the
@@ -106,6 +110,11 @@ double ora_ssc_q(int R, const int32_t* IE, double dB, double Emean);
 double ora_msc_q(int w, int rg, const uint8_t* active, const int32_t* IE, double dB,
                  double Emean);
 double ora_lnq(double dB, double Emean, double q, int Rprev);
+/* CalcOverlap (SSC:317-334, MSC:477-499): energy-histogram overlap of the current
+ * population for a candidate step dB. */
+double ora_ssc_overlap(int R, const int32_t* IE, int Rinit, double dB, double Emean);
+double ora_msc_overlap(int w, int rg, const uint8_t* active, const int32_t* IE, int Rinit,
+                       int R, double dB, double Emean);
 void ora_ssc_averages(int R, const int32_t* IE, const int32_t* M, double out[5]);
 void ora_msc_averages(int w, int rg, const uint8_t* active, const int32_t* IE,
                       const int32_t* M, double out[5]);

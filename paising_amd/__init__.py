@@ -31,7 +31,9 @@ class Params(ctypes.Structure):
         ("Binit", ctypes.c_double), ("Bfin", ctypes.c_double), ("seed", ctypes.c_uint64),
         ("initial_sequence", ctypes.c_uint64), ("uniform", ctypes.c_int32),
         ("max_steps", ctypes.c_int32), ("trace", ctypes.c_int32), ("slice_bits", ctypes.c_int32),
-        ("threads", ctypes.c_int32), ("balance", ctypes.c_int32),
+        ("threads", ctypes.c_int32), ("balance", ctypes.c_int32), ("adaptive", ctypes.c_int32),
+        ("schedule_len", ctypes.c_int32), ("min_overlap", ctypes.c_double),
+        ("max_overlap", ctypes.c_double), ("schedule", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -118,9 +120,19 @@ def _ptr(a):
 
 def make_params(L=64, coding=MSC, msbits=32, Rinit=20000, eqsweeps=100, dB=0.005, Binit=0.0,
                 Bfin=1.0, seed=100, initial_sequence=0, uniform=UNIFORM_CUDA101, max_steps=0,
-                trace=0, slice_bits=0, threads=0, balance=0):
-    return Params(L, coding, msbits, Rinit, eqsweeps, dB, Binit, Bfin, seed & (2**64 - 1),
-                  initial_sequence, uniform, max_steps, trace, slice_bits, threads, balance)
+                trace=0, slice_bits=0, threads=0, balance=0, adaptive=0, min_overlap=0.0,
+                max_overlap=0.0, schedule=None):
+    """pa_params.  adaptive=1: the reference's adaptive beta step; schedule: an explicit
+    list of betas (Binit first) to replay, e.g. run 0's adaptive steps for later runs."""
+    p = Params(L, coding, msbits, Rinit, eqsweeps, dB, Binit, Bfin, seed & (2**64 - 1),
+               initial_sequence, uniform, max_steps, trace, slice_bits, threads, balance,
+               adaptive, 0, min_overlap, max_overlap, None)
+    if schedule is not None:
+        arr = (ctypes.c_double * len(schedule))(*schedule)
+        p._schedule_keep = arr  # the library copies it at creation; keep it alive until then
+        p.schedule = ctypes.cast(arr, ctypes.POINTER(ctypes.c_double))
+        p.schedule_len = len(schedule)
+    return p
 
 
 def autotune(params):
@@ -193,6 +205,14 @@ class Run:
 
     def sync(self):
         _check(lib().pa_run_sync(self.h))
+
+    def steps_done(self):
+        """(completed steps, steps of the schedule); syncs.  For an adaptive run the total is
+        the table capacity until the schedule has passed Bfin."""
+        d = ctypes.c_int(0)
+        t = ctypes.c_int(0)
+        _check(lib().pa_run_steps_done(self.h, ctypes.byref(d), ctypes.byref(t)))
+        return d.value, t.value
 
     def set_timing(self, on=True):
         _check(lib().pa_run_set_timing(self.h, int(on)))

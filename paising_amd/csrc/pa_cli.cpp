@@ -6,6 +6,8 @@
 //   -L size   (PAisingSSC.cu:27, default 64)
 //   -w bits   (PAisingMSC.cu:34 MSbits, 32 or 64, default 32; MSC only)
 //   -u conv   uniform-double convention: 0 = CUDA >= 10.1 (default), 1 = CUDA <= 10.0
+//   -a        adaptive inverse-temperature step (the reference's AdaptiveStep build,
+//             PAisingSSC.cu:23,40-43,406-410,644-660); later -M runs replay run 0's steps
 // The GPU work goes through the C ABI of paising.h (pa_run_all per run).
 #include <sys/stat.h>
 
@@ -81,7 +83,8 @@ void usage(const char* prog) {
 #if PA_CLI_CODING == 1
             << " -w bits            ( bits            = replicas per multi-spin word, 32 or 64 )\n"
 #endif
-            << " -u convention      ( convention      = uniform-double mapping: 0 CUDA>=10.1, 1 CUDA<=10.0 )\n";
+            << " -u convention      ( convention      = uniform-double mapping: 0 CUDA>=10.1, 1 CUDA<=10.0 )\n"
+            << " -a                 ( adaptive inverse temperature step, MinOverlap 0.85, MaxOverlap 0.87 )\n";
 }
 
 }  // namespace
@@ -94,10 +97,10 @@ int main(int argc, char** argv) {
   long long seed_opt = (long long)time(NULL);
   unsigned int EQsweeps = 100;
   double Bfin = 1.0, dBinit = 0.005;
-  int Rinit = 20000, runs = 1, OutputPrecision = 11, L = 64, bits = 32, uconv = 0;
+  int Rinit = 20000, runs = 1, OutputPrecision = 11, L = 64, bits = 32, uconv = 0, adaptive = 0;
   int optc, opti;
   double optf;
-  while ((optc = get_opt(argc, argv, msc ? "R:t:d:f:M:s:P:o:L:w:u:?" : "R:t:d:f:M:s:P:o:L:u:?")) != -1) {
+  while ((optc = get_opt(argc, argv, msc ? "R:t:d:f:M:s:P:o:L:w:u:a?" : "R:t:d:f:M:s:P:o:L:u:a?")) != -1) {
     switch (optc) {
       case 'R': opti = atoi(opt_arg); if (opti) Rinit = opti; break;
       case 't': opti = atoi(opt_arg); EQsweeps = opti; break;
@@ -114,6 +117,7 @@ int main(int argc, char** argv) {
       case 'L': opti = atoi(opt_arg); if (opti) L = opti; break;
       case 'w': opti = atoi(opt_arg); if (opti) bits = opti; break;
       case 'u': uconv = atoi(opt_arg); break;
+      case 'a': adaptive = 1; break;
       case '?': usage(prog); return 1;
     }
   }
@@ -125,9 +129,14 @@ int main(int argc, char** argv) {
   // (PAisingMSC.cu:549); both reach curand_init as unsigned long long.
   const int seed_int = (int)seed_opt;
   const unsigned long long seed = msc ? (unsigned long long)seed_opt : (unsigned long long)(long long)seed_int;
-  if (!optdir)
-    snprintf(dataDir, sizeof dataDir, "./data%s_L%d_R%d_EqSw%d_dB%f/", msc ? "MSC" : "SSC", L, Rinit,
-             EQsweeps, dBinit);
+  if (!optdir) {
+    if (adaptive)  // PAisingSSC.cu:406-408: no dB in the name of an adaptive run
+      snprintf(dataDir, sizeof dataDir, "./data%s_L%d_R%d_EqSw%d/", msc ? "MSC" : "SSC", L, Rinit,
+               EQsweeps);
+    else
+      snprintf(dataDir, sizeof dataDir, "./data%s_L%d_R%d_EqSw%d_dB%f/", msc ? "MSC" : "SSC", L,
+               Rinit, EQsweeps, dBinit);
+  }
   mkdir(dataDir, 0777);
 
   pa_params p;
@@ -171,16 +180,29 @@ int main(int argc, char** argv) {
   double totPop = 0.0;
   unsigned long long IS = 0;
   std::vector<pa_row> rows(10000);
+  std::vector<double> sched;  // adaptive: run 0's betas, replayed by later runs (SSC:662)
+  p.adaptive = adaptive;
   for (int r = 0; r < runs; ++r) {
     p.initial_sequence = IS;
+    if (adaptive && r > 0) {
+      p.adaptive = 0;
+      p.schedule = sched.data();
+      p.schedule_len = (int)sched.size();
+    }
     int nrows = 0;
     uint64_t is_end = 0;
     const int rc = pa_run_all(&p, rows.data(), (int)rows.size(), &nrows, &is_end);
     if (rc == PA_ERR_NBMAX) {
-      fprintf(stderr,
-              "Error: number of temperature steps exceeds nBmax=%d.\n Please consider increasing the "
-              "inverse temperature step or increasing the value of nBmax.\n",
-              10000);
+      if (adaptive)
+        fprintf(stderr,
+                "Error: number of temperature steps exceeds nBmax=%d.\n Please consider increasing the "
+                "population size or decreasing the value of MinOverlap or increasing the value of nBmax.\n",
+                10000);
+      else
+        fprintf(stderr,
+                "Error: number of temperature steps exceeds nBmax=%d.\n Please consider increasing the "
+                "inverse temperature step or increasing the value of nBmax.\n",
+                10000);
       return 1;
     }
     if (rc) {
@@ -188,6 +210,8 @@ int main(int argc, char** argv) {
       return rc;
     }
     IS = is_end;
+    if (adaptive && r == 0)
+      for (int i = 0; i < nrows; ++i) sched.push_back(rows[i].B);
     for (int i = 1; i < nrows; ++i) totPop += rows[i].R;
     char str[700];
     snprintf(str, sizeof str, "%sPA_results_run_%d.dat", dataDir, r);

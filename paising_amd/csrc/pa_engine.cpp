@@ -100,6 +100,10 @@ struct RunHost {
   int rs_split = 1;
   int rs_lp = 1;
   std::vector<double> B;
+  // adaptive schedule: B grows as steps are decided; deltaBeta carries over (SSC:645-660)
+  bool adaptive = false;
+  bool adaptive_done = false;
+  double deltaBeta = 0.0;
   size_t footprint = 0;
   // device memory (one allocation carved into pieces)
   void* arena = nullptr;
@@ -171,12 +175,40 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma) {
   if (!(p.dB > 0.0) || p.Binit < 0.0) return fail(PA_ERR_INVALID, "dB must be > 0, Binit >= 0");
   if (p.uniform != PA_UNIFORM_CUDA101 && p.uniform != PA_UNIFORM_CUDA100)
     return fail(PA_ERR_INVALID, "uniform");
+  if (p.schedule_len < 0 || (p.schedule_len > 0 && !p.schedule))
+    return fail(PA_ERR_INVALID, "schedule_len > 0 needs a schedule");
+  if (p.adaptive && p.schedule_len > 0)
+    return fail(PA_ERR_INVALID, "adaptive and an explicit schedule exclude each other");
+  if (p.adaptive) {
+    if (p.min_overlap == 0.0 && p.max_overlap == 0.0) {
+      p.min_overlap = 0.85;  // MinOverlap, SSC:41
+      p.max_overlap = 0.87;  // MaxOverlap, SSC:42
+    }
+    if (!(p.min_overlap > 0.0 && p.min_overlap < p.max_overlap && p.max_overlap <= 1.0))
+      return fail(PA_ERR_INVALID, "adaptive step needs 0 < min_overlap < max_overlap <= 1");
+  }
+  p.schedule = nullptr;  // copied below; the caller's array is not kept
   h.p = p;
   h.w = p.coding == PA_SSC ? 1 : p.msbits;
   h.N = p.L * p.L;
-  h.B = schedule(p.Binit, p.dB, p.Bfin, p.max_steps);
-  h.n_steps = (int)h.B.size() - 1;
-  if (h.n_steps + 1 >= kNBmax)
+  if (p_in.schedule_len > 0) {
+    // replay of an explicit schedule (later runs of an adaptive -M series, SSC:662)
+    h.B.assign(p_in.schedule, p_in.schedule + p_in.schedule_len);
+    if (p.max_steps > 0 && (int)h.B.size() > p.max_steps + 1) h.B.resize(p.max_steps + 1);
+    h.n_steps = (int)h.B.size() - 1;
+  } else if (p.adaptive) {
+    // steps are decided one at a time after each step; the tables hold up to nBmax steps
+    h.adaptive = true;
+    h.deltaBeta = p.dB;
+    h.B = {p.Binit};
+    if (p.Binit + p.dB <= p.Bfin) h.B.push_back(p.Binit + p.dB);
+    h.adaptive_done = h.B.size() == 1;
+    h.n_steps = h.adaptive_done ? 0 : (p.max_steps > 0 ? std::min(p.max_steps, kNBmax - 1) : kNBmax - 1);
+  } else {
+    h.B = schedule(p.Binit, p.dB, p.Bfin, p.max_steps);
+    h.n_steps = (int)h.B.size() - 1;
+  }
+  if (h.n_steps + 1 > kNBmax || (!h.adaptive && h.n_steps + 1 >= kNBmax))
     return fail(PA_ERR_NBMAX, "Error: number of temperature steps exceeds nBmax=10000.");
   h.R0 = p.Rinit;
   const double margin = cap_sigma * std::sqrt((double)p.Rinit) + 64.0;
@@ -255,6 +287,21 @@ size_t layout(RunHost& h, uint8_t* base) {
   return o;
 }
 
+// Acceptance thresholds of one step (3 words per step in the device table).
+void step_thresholds(const RunHost& h, double b, uint64_t* t) {
+  if (h.p.coding == PA_SSC) {
+    // T[idxT0 + 2 idxT1] = min(1, exp(-2 (2 idxT0 - 1)(2 idxT1 - 4) B)), SSC:512-516:
+    // dE <= 0 -> 1, dE = 4 -> exp(-4B), dE = 8 -> exp(-8B)
+    t[0] = ssc_threshold(1.0, h.p.uniform);
+    t[1] = ssc_threshold(std::min(1.0, std::exp(-4 * b)), h.p.uniform);
+    t[2] = ssc_threshold(std::min(1.0, std::exp(-8 * b)), h.p.uniform);
+  } else {
+    t[0] = (uint32_t)std::ceil(4294967296. * std::exp(-4 * b));  // MSC:674
+    t[1] = (uint32_t)std::ceil(4294967296. * std::exp(-8 * b));  // MSC:675
+    t[2] = 0;
+  }
+}
+
 // Allocate and upload a planned run's state (tables, initial RunState); no kernels.
 int materialize(RunHost& h) {
   h.footprint = layout(h, nullptr);
@@ -269,23 +316,16 @@ int materialize(RunHost& h) {
   d.rs_lp = h.rs_lp;
   d.rs_split = h.rs_split;
   d.ld2_magic = (uint32_t)((1ull << 32) / (uint64_t)d.Ld2 + 1);
-  std::vector<double> dbeta(h.n_steps + 1, 0.0);
+  std::vector<double> beta(h.n_steps + 1, 0.0), dbeta(h.n_steps + 1, 0.0);
   std::vector<uint64_t> thr((size_t)(h.n_steps + 1) * 3, 0);
-  for (int i = 1; i <= h.n_steps; ++i) {
-    const double b = h.B[i];
+  beta[0] = h.B[0];
+  const int known = std::min(h.n_steps, (int)h.B.size() - 1);  // adaptive: step 1 only
+  for (int i = 1; i <= known; ++i) {
+    beta[i] = h.B[i];
     dbeta[i] = h.B[i] - h.B[i - 1];
-    if (h.p.coding == PA_SSC) {
-      // T[idxT0 + 2 idxT1] = min(1, exp(-2 (2 idxT0 - 1)(2 idxT1 - 4) B)), SSC:512-516:
-      // dE <= 0 -> 1, dE = 4 -> exp(-4B), dE = 8 -> exp(-8B)
-      thr[3 * i + 0] = ssc_threshold(1.0, h.p.uniform);
-      thr[3 * i + 1] = ssc_threshold(std::min(1.0, std::exp(-4 * b)), h.p.uniform);
-      thr[3 * i + 2] = ssc_threshold(std::min(1.0, std::exp(-8 * b)), h.p.uniform);
-    } else {
-      thr[3 * i + 0] = (uint32_t)std::ceil(4294967296. * std::exp(-4 * b));  // MSC:674
-      thr[3 * i + 1] = (uint32_t)std::ceil(4294967296. * std::exp(-8 * b));  // MSC:675
-    }
+    step_thresholds(h, h.B[i], &thr[3 * i]);
   }
-  HIPCHECK(hipMemcpy((void*)d.beta, h.B.data(), (h.n_steps + 1) * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy((void*)d.beta, beta.data(), (h.n_steps + 1) * 8, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy((void*)d.dbeta, dbeta.data(), (h.n_steps + 1) * 8, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy((void*)d.thr, thr.data(), thr.size() * 8, hipMemcpyHostToDevice));
   HIPCHECK(hipMemset(d.out, 0, (h.n_steps + 1) * sizeof(StepOut)));
@@ -589,6 +629,90 @@ struct Batch {
     return PA_OK;
   }
 
+  // Adaptive schedule (plain single-run batches): decide B_i from the population after
+  // step i-1 by the reference's growth + bisection search on the histogram overlap
+  // (SSC:644-660, MSC:795-811), each overlap evaluated on the device (k_qgroup, k_qpart,
+  // k_overlap; CalcOverlap SSC:317-334) with the block sums added on the host in launch
+  // order; then write the step's beta, dB and thresholds.  Sets h.adaptive_done when the
+  // schedule has passed Bfin.
+  double stage_db = 0.0;
+  std::vector<double> stage_part;
+  std::vector<uint64_t> stage_thr;
+  double stage_beta[2] = {0.0, 0.0};
+  int adaptive_next(RunHost& h, int i) {
+    if (i >= kNBmax)
+      return fail(PA_ERR_NBMAX,
+                  "Error: number of temperature steps exceeds nBmax=10000.\n Please consider "
+                  "increasing the population size or decreasing the value of MinOverlap or "
+                  "increasing the value of nBmax.");
+    HIPCHECK(hipStreamSynchronize(stream));
+    RunState st;
+    HIPCHECK(hipMemcpy(&st, h.desc.st, sizeof st, hipMemcpyDeviceToHost));
+    if (st.overflow) return fail(PA_ERR_CAPACITY, "population outgrew its buffers");
+    if (st.err) return fail(PA_ERR_HIP, "device error bits " + std::to_string(st.err));
+    if (st.step != i) return fail(PA_ERR_STATE, "adaptive step out of sequence");
+    const int n = (int)runs.size();
+    const int R = st.R, w = h.w;
+    const int nblk = h.p.coding == PA_SSC ? ceil_div(R, kRedBlock)
+                                          : ceil_div(ceil_div(R, w), kRedBlock) * w;
+    stage_part.resize((size_t)nblk * 5);
+    int rc = PA_OK;
+    auto overlap = [&](double dB) -> double {
+      stage_db = dB;
+      hipError_t e = hipMemcpyAsync((void*)(h.desc.dbeta + i), &stage_db, 8, hipMemcpyHostToDevice, stream);
+      if (e == hipSuccess && h.p.coding == PA_MSC)
+        e = pa_launch_step(8, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n);
+      if (e == hipSuccess) e = pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n);
+      if (e == hipSuccess) e = pa_launch_step(12, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(stage_part.data(), h.desc.avblk, stage_part.size() * 8, hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) { rc = fail(PA_ERR_HIP, hipGetErrorString(e)); return 0.0; }
+      double acc = 0.0;
+      for (int c = 0; c < nblk; ++c) acc += stage_part[(size_t)c * 5];
+      return acc / R;
+    };
+    const double Bprev = h.B[i - 1], Bfin = h.p.Bfin;
+    const double MinOverlap = h.p.min_overlap, MaxOverlap = h.p.max_overlap;
+    double ovl, dBmin = 0, dBmax = h.deltaBeta, dBmean;
+    for (int it = 0;; ++it) {
+      if (it > 100000) return fail(PA_ERR_STATE, "adaptive step: growth did not stop");
+      ovl = overlap(dBmax);
+      if (rc) return rc;
+      if ((ovl >= MaxOverlap) && (Bprev + dBmax < Bfin)) dBmax *= 1.1;
+      else break;
+    }
+    if (ovl >= MinOverlap) {
+      dBmean = dBmax;
+    } else {
+      for (int it = 0;; ++it) {
+        if (it > 100000) return fail(PA_ERR_STATE, "adaptive step: bisection did not converge");
+        dBmean = 0.5 * (dBmin + dBmax);
+        ovl = overlap(dBmean);
+        if (rc) return rc;
+        if (ovl < MinOverlap) dBmax = dBmean;
+        else if (ovl >= MaxOverlap) dBmin = dBmean;
+        else break;
+      }
+    }
+    if ((Bprev < Bfin) && (Bprev + dBmean > Bfin)) h.deltaBeta = Bfin - Bprev;
+    else h.deltaBeta = dBmean;
+    const double Bi = Bprev + h.deltaBeta;
+    if (!(Bi <= Bfin)) {
+      h.adaptive_done = true;
+      return PA_OK;
+    }
+    h.B.push_back(Bi);
+    stage_beta[0] = Bi;
+    stage_beta[1] = Bi - Bprev;
+    stage_thr.assign(3, 0);
+    step_thresholds(h, Bi, stage_thr.data());
+    HIPCHECK(hipMemcpyAsync((void*)(h.desc.beta + i), &stage_beta[0], 8, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync((void*)(h.desc.dbeta + i), &stage_beta[1], 8, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync((void*)(h.desc.thr + 3 * i), stage_thr.data(), 24, hipMemcpyHostToDevice, stream));
+    return PA_OK;
+  }
+
   int run_steps(int k) {
     for (int i = 0; i < k; ++i) {
       if (use_graph) {
@@ -770,6 +894,23 @@ int pa_run_create(const pa_params* p, pa_run** out) { return run_create_sigma(p,
 
 int pa_run_step(pa_run* r, int nsteps) {
   if (!r) return fail(PA_ERR_INVALID, "null run");
+  RunHost& h = *r->host;
+  if (h.adaptive) {
+    // one step at a time: each step's beta depends on the population after the last one
+    h.results_valid = false;
+    for (int todo = nsteps < 0 ? INT32_MAX : nsteps; todo > 0; --todo) {
+      if (h.adaptive_done || r->batch.steps_launched >= h.n_steps) break;
+      const int i = r->batch.steps_launched + 1;
+      if (i >= 2) {
+        int rc = r->batch.adaptive_next(h, i);
+        if (rc) return rc;
+        if (h.adaptive_done) break;
+      }
+      int rc = r->batch.run_steps(1);
+      if (rc) return rc;
+    }
+    return PA_OK;
+  }
   const int remaining = r->host->n_steps - r->batch.steps_launched;
   const int k = nsteps < 0 ? remaining : std::min(nsteps, remaining);
   r->host->results_valid = false;
@@ -793,7 +934,9 @@ int pa_run_steps_done(pa_run* r, int* done, int* total) {
   int rc = pa_run_sync(r);
   if (rc) return rc;
   if (done) *done = r->host->st_h.step - 1;
-  if (total) *total = r->host->n_steps;
+  // adaptive: the number of steps is known once the schedule has passed Bfin
+  if (total) *total = r->host->adaptive && r->host->adaptive_done ? (int)r->host->B.size() - 1
+                                                                 : r->host->n_steps;
   return PA_OK;
 }
 
@@ -984,6 +1127,7 @@ int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out
   int rc;
   if ((rc = b->batch.create_stream())) return rc;
   for (int i = 0; i < nruns; ++i) {
+    if (p[i].adaptive) return fail(PA_ERR_INVALID, "adaptive runs take the plain path (pa_run_*)");
     auto h = std::make_unique<RunHost>();
     if ((rc = plan_run(p[i], *h, 10.0))) return rc;
     if ((rc = materialize(*h))) return rc;
@@ -1097,6 +1241,7 @@ int pa_server_set_chunk(pa_server* s, int steps_per_tick, int use_graph) {
 
 int pa_server_submit(pa_server* s, const pa_params* p, int64_t* ticket) {
   if (!s || !p || !ticket) return fail(PA_ERR_INVALID, "null argument");
+  if (p->adaptive) return fail(PA_ERR_INVALID, "adaptive runs take the plain path (pa_run_*)");
   Request rq;
   rq.ticket = s->next_ticket++;
   rq.p = *p;

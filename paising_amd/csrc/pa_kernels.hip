@@ -476,6 +476,62 @@ __global__ __launch_bounds__(1024) void k_tau(const RunDesc* __restrict__ runs,
 }
 
 // ============================================================================================
+// k_overlap: energy-histogram overlap of the current population for the candidate step
+// dB = dbeta[step] (adaptive step; HistogramOverlap + CalcOverlap, SSC:308-334 and
+// MSC:466-499): q from the k_qgroup/k_qpart partials as in k_tau, lnQ, then per block of
+// 1024 replicas (SSC) or (1024 groups, lane) (MSC) the tree sum of
+// min(1, Rinit/R exp(-dB E - lnQ)) into avblk[5 c].  The host adds the block sums in launch
+// order (the reference adds them with atomicAdd).
+// ============================================================================================
+__global__ __launch_bounds__(1024) void k_overlap(const RunDesc* __restrict__ runs,
+                                                  const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKAvg * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (st->overflow) return;
+  const int c = blockIdx.x - off[r];
+  const int R = st->R, w = d.w, buf = st->buf, step = st->step;
+  int nb, b, by;
+  if (d.coding == kSSC) {
+    nb = ceil_div(R, kRB);
+    if (c >= nb) return;
+    b = c; by = 0;
+  } else {
+    nb = ceil_div(ceil_div(R, w), kRB);
+    if (c >= nb * w) return;
+    by = c / nb; b = c % nb;
+  }
+  __shared__ double sh[32];
+  const int nitems = (d.coding == kSSC) ? R : ceil_div(R, w);
+  const int nqb = ceil_div(nitems, kRB);
+  double acc = 0.0;
+  for (int j = 0; j * kRB < nitems; j += kRB) {
+    const int k = threadIdx.x + j;
+    const double f = (k < nqb) ? d.qblk[k] : 0.0;
+    acc += ref_block_sum(f, sh);
+    __syncthreads();
+  }
+  // the block sum is valid in thread 0 only: broadcast lnQ
+  const double dB = d.dbeta[step];
+  __shared__ double lnq_sh;
+  if (threadIdx.x == 0) lnq_sh = -(dB)*st->Emean + log_fp64(acc) - log_fp64((double)R);
+  __syncthreads();
+  const double lnQ = lnq_sh;
+  const double ratio = (double)d.Rinit / (double)R;
+  const int idx = b * kRB + threadIdx.x;
+  double v = 0.0;
+  if (d.coding == kSSC) {
+    if (idx < R) v = ratio * exp_fp64(fma_(-dB, (double)d.IE[idx], -lnQ));
+  } else if (idx < ceil_div(R, w) && ((d.amask[buf][idx] >> by) & 1)) {
+    v = ratio * exp_fp64(fma_(-dB, (double)d.IE[(size_t)idx * w + by], -lnQ));
+  }
+  v = v < 1.0 ? v : 1.0;
+  const double sblk = ref_block_sum(v, sh);
+  if (threadIdx.x == 0) d.avblk[(size_t)c * 5] = sblk;
+}
+
+// ============================================================================================
 // k_scan: exclusive prefix of Roff in population order (CalcParSum), R_i, capacity check.
 // ============================================================================================
 __global__ __launch_bounds__(1024) void k_scan(const RunDesc* __restrict__ runs,
@@ -1300,6 +1356,7 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 8: hipLaunchKernelGGL(k_qgroup, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 9: hipLaunchKernelGGL(k_parents, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 11: return launch_sweep_msc_bal(w, sw, nt, nblocks, lds, s, runs, nruns);
+    case 12: hipLaunchKernelGGL(k_overlap, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 10:
       if (w == 32) hipLaunchKernelGGL(k_resample_msc<uint32_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
       else hipLaunchKernelGGL(k_resample_msc<uint64_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);

@@ -8,11 +8,11 @@ UMAP = {pa.UNIFORM_CUDA101: oracle.U32_OPEN_CLOSED, pa.UNIFORM_CUDA100: oracle.U
 
 
 def oracle_rows(L, coding, msbits, Rinit, eqsweeps, dB, Bfin, seed, steps, uniform=0,
-                initial_sequence=0, threads=0):
+                initial_sequence=0, threads=0, **kw):
     rc, runs, _, _ = oracle.run(L=L, coding=coding, msbits=msbits, Rinit=Rinit,
                                 eqsweeps=eqsweeps, dB=dB, Bfin=Bfin, seed=seed,
                                 initial_sequence=initial_sequence, umap=UMAP[uniform],
-                                max_steps=steps, threads=threads)
+                                max_steps=steps, threads=threads, **kw)
     assert rc == 0
     return runs[0]
 
