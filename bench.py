@@ -168,6 +168,35 @@ def bench_config3(args):
             "R": [r.R for r in rows]}
 
 
+def bench_adaptive(args):
+    """Adaptive inverse-temperature step (the reference's AdaptiveStep build, MinOverlap 0.85,
+    MaxOverlap 0.87) at L=64, R=20000, EqSw=100, beta 0->1, seed 100, whole run: each step's
+    beta comes from a growth + bisection search on the energy-histogram overlap, evaluated on
+    the device with a host sync per evaluation."""
+    out = {"metric": "spin-flip attempts/ns, adaptive beta step, L=64 R=2e4 (whole run)",
+           "unit": "spin-flip attempts/ns", "n_gpus": 1, "higher_is_better": True,
+           "config": {"workload": "adaptive PA, L=64, R=20000, EqSw=100, beta 0->1, seed 100"}}
+    for name, coding in (("msc32", pa.MSC), ("ssc", pa.SSC)):
+        p = pa.make_params(L=L, coding=coding, msbits=32, Rinit=RINIT, eqsweeps=EQSW, dB=DB,
+                           Bfin=1.0, seed=SEED, adaptive=1)
+        r = pa.Run(p)
+        r.set_timing(True)
+        t0 = time.perf_counter()
+        r.step(-1)
+        r.sync()
+        wall = time.perf_counter() - t0
+        rs_ms, sw_ms, nsteps = r.kernel_times()
+        rows = r.results()
+        r.close()
+        att = EQSW * N * sum(x.R for x in rows[1:])
+        out[name] = {"value": round(att / (wall * 1e9), 2), "beta_steps": len(rows) - 1,
+                     "wall_s": round(wall, 3), "sweep_s": round(sw_ms / 1e3, 3),
+                     "non_sweep_s_per_step": round((wall - sw_ms / 1e3) / max(1, nsteps), 5),
+                     "last_beta": rows[-1].B}
+    out["value"] = out["msc32"]["value"]
+    return out
+
+
 def batch_mix(nruns=64, seed=2026, steps=0):
     """The batch mix: 64 independent runs of uneven L, R and EqSw, both codings, dB=0.005, beta 0->1."""
     import random
@@ -269,7 +298,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "copy", "config3"])
+    ap.add_argument("--mode", default="plain",
+                    choices=["plain", "batch", "serve", "copy", "config3", "adaptive"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=24, help="beta steps in the CPU baseline sample")
     ap.add_argument("--no-tune", action="store_true",
@@ -298,6 +328,9 @@ def main():
         return
     if args.mode == "config3":
         print(json.dumps(bench_config3(args)), flush=True)
+        return
+    if args.mode == "adaptive":
+        print(json.dumps(bench_adaptive(args)), flush=True)
         return
     name, cu, _ = pa.device_info()
     clock = device_clock_hz()

@@ -65,6 +65,10 @@ typedef struct pa_params {
   double min_overlap;        /* adaptive: MinOverlap, 0 = 0.85 (PAisingSSC.cu:41)          */
   double max_overlap;        /* adaptive: MaxOverlap, 0 = 0.87 (PAisingSSC.cu:42)          */
   const double* schedule;    /* explicit schedule (B[0] = Binit), copied at creation      */
+  int32_t mhr;               /* 1 = accumulate the energy histogram over all steps for    */
+                             /* multi-histogram reweighting (MHR build, PAisingSSC.cu:22, */
+                             /* 294-304,496-501,608-611)                                  */
+  int32_t pad1;
 } pa_params;
 
 /* One row of PA_results_run_<r>.dat (PAisingSSC.cu:679-690) and the step internals. */
@@ -136,6 +140,20 @@ int pa_autotune_clear(void);
 
 /* Whole run in one call: create, all steps, results (a reference run, PAisingSSC.cu:481-692). */
 int pa_run_all(const pa_params* p, pa_row* rows, int max_rows, int* nrows, uint64_t* final_is);
+/* pa_run_all that also returns the run's energy histogram (p->mhr = 1): hist[k] counts the
+ * replicas with energy 4k - 2N summed over the initial population and every step
+ * (ShistE, PAisingSSC.cu:496-501,608-611); nbins must be at least N + 1. */
+int pa_run_all_hist(const pa_params* p, pa_row* rows, int max_rows, int* nrows,
+                    uint64_t* final_is, int32_t* hist, int nbins);
+/* The energy histogram of a run created with p->mhr = 1 (after pa_run_sync). */
+int pa_run_histogram(pa_run* r, int32_t* hist, int nbins);
+/* Multi-histogram reweighting of one run (PAisingSSC.cu:695-764, host FP64): nB rows with
+ * inverse temperatures B, populations R, dimensionless free energies BF (unnormalised:
+ * BF[0] = -N ln 2, BF[i] = BF[0] - sum of lnQ; updated in place by every iteration, as the
+ * reference does), the histogram of N + 1 bins, niter iterations.  out receives nB rows of
+ * 1 + 3 niter doubles: B, then per iteration E/N, C/N, BF/N (PA_MHR_results_run_<r>.dat). */
+int pa_mhr_analysis(int N, int nB, const double* B, double* BF, const int32_t* R,
+                    const int32_t* hist, int niter, double* out);
 
 /* ---- batched runs (no reference counterpart: many independent runs per launch) ---- */
 int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out);

@@ -30,7 +30,7 @@ class Params(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("initial_sequence", ctypes.c_uint64),
         ("umap", ctypes.c_int32), ("max_steps", ctypes.c_int32), ("threads", ctypes.c_int32),
         ("adaptive", ctypes.c_int32), ("pad0", ctypes.c_int32), ("min_overlap", ctypes.c_double),
-        ("max_overlap", ctypes.c_double),
+        ("max_overlap", ctypes.c_double), ("mhr", ctypes.c_int32), ("mhr_iter", ctypes.c_int32),
     ]
 
 
@@ -65,6 +65,8 @@ def lib():
                              ctypes.c_uint32)
         sig = {
             "ora_run": (I, [ctypes.POINTER(Params), P, I, P, P, P]),
+            "ora_run_ex": (I, [ctypes.POINTER(Params), P, I, P, P, P, P, P]),
+            "ora_mhr_analysis": (None, [I, I, P, P, P, P, I, P]),
             "ora_philox": (None, [P, P, P]),
             "ora_stream_word": (U32, [U64, U64, U64]),
             "ora_uniform": (D, [U32, U32, I]),
@@ -107,21 +109,46 @@ def run(L=64, coding=SSC, msbits=32, Rinit=20000, eqsweeps=100, dB=0.005, Binit=
         Bfin=1.0, runs=1, seed=100, initial_sequence=0, umap=U_DEFAULT, max_steps=0,
         threads=0, max_rows=10000, adaptive=0, min_overlap=0.0, max_overlap=0.0):
     """Run the oracle; returns (rc, list of per-run row lists, elapsed_s, total_pop)."""
+    rc, out, el, tot, _, _ = run_mhr(L, coding, msbits, Rinit, eqsweeps, dB, Binit, Bfin, runs,
+                                     seed, initial_sequence, umap, max_steps, threads, max_rows,
+                                     adaptive, min_overlap, max_overlap, mhr=0)
+    return rc, out, el, tot
+
+
+def run_mhr(L=64, coding=SSC, msbits=32, Rinit=20000, eqsweeps=100, dB=0.005, Binit=0.0,
+            Bfin=1.0, runs=1, seed=100, initial_sequence=0, umap=U_DEFAULT, max_steps=0,
+            threads=0, max_rows=10000, adaptive=0, min_overlap=0.0, max_overlap=0.0, mhr=1,
+            mhr_iter=1):
+    """run() of the MHR build; also returns per-run energy histograms (N+1 ints) and
+    PA_MHR_results rows ([B, E/N, C/N, BF/N per iteration])."""
     p = Params(L, coding, msbits, Rinit, eqsweeps, dB, Binit, Bfin, runs,
                seed & (2**64 - 1), initial_sequence, umap, max_steps, threads, adaptive, 0,
-               min_overlap, max_overlap)
+               min_overlap, max_overlap, mhr, mhr_iter)
+    N = L * L
+    cols = 1 + 3 * max(1, mhr_iter)
+    hist = np.zeros(runs * (N + 1), dtype=np.int32) if mhr else None
+    mrows = np.zeros(runs * max_rows * cols, dtype=np.float64) if mhr else None
     rows = (Row * (max_rows * runs))()
     nrows = np.zeros(runs, dtype=np.int32)
     el = ctypes.c_double(0)
     tot = ctypes.c_double(0)
-    rc = lib().ora_run(ctypes.byref(p), rows, max_rows, _ptr(nrows), ctypes.byref(el),
-                       ctypes.byref(tot))
+    rc = lib().ora_run_ex(ctypes.byref(p), rows, max_rows, _ptr(nrows), ctypes.byref(el),
+                          ctypes.byref(tot), _ptr(hist) if mhr else None,
+                          _ptr(mrows) if mhr else None)
     out = [[rows[r * max_rows + i] for i in range(int(nrows[r]))] for r in range(runs)]
-    return rc, out, el.value, tot.value
+    hists = [hist[r * (N + 1):(r + 1) * (N + 1)].tolist() for r in range(runs)] if mhr else None
+    mhr_rows = ([[mrows[(r * max_rows + i) * cols:(r * max_rows + i + 1) * cols].tolist()
+                  for i in range(int(nrows[r]))] for r in range(runs)] if mhr else None)
+    return rc, out, el.value, tot.value, hists, mhr_rows
 
 
 def row_fields(row):
     return [row.B, row.E, row.C, row.M, row.M2, row.M4, row.BF, row.S, row.R, row.lnQ]
+
+
+def format_mhr_rows(rows, precision=11):
+    """PA_MHR_results_run_r.dat lines (SSC:776-786): every value followed by a space."""
+    return "".join("".join("%.*g " % (precision, v) for v in r) + "\n" for r in rows)
 
 
 def format_rows(rows, precision=11):

@@ -743,6 +743,75 @@ int adaptive_step(const ora_params* p, double Bprev, double& deltaBeta, F ovl) {
   return 0;
 }
 
+/* UpdateShistE, SSC:296-302 / MSC:454-460: histogram bin (2N + E)/4 per replica. */
+void hist_add(int N, int64_t n, const int32_t* IE, const uint8_t* active, std::vector<int32_t>& H) {
+  for (int64_t k = 0; k < n; ++k)
+    if (!active || active[k]) H[(2 * N + IE[k]) / 4] += 1;
+}
+
+/* Multi-histogram reweighting, SSC:705-764 (MSC:855-914).  Ei[k] = 4k - 2N (SSC:461-463).
+ * out: nB rows of 1 + 3 niter doubles (B, then E/N, C/N, BF/N per iteration).  BF is
+ * overwritten with the reweighted free energies of each iteration, as in the reference. */
+void mhr_analysis(int N, int nB, const double* B, double* BF, const int32_t* R,
+                  const int32_t* hist, int niter, double* out) {
+  const int K = N + 1, cols = 1 + 3 * niter;
+  std::vector<int> Ei(K);
+  for (int k = 0; k < K; ++k) Ei[k] = 4 * k - 2 * N;
+  std::vector<double> lnOmega(K), Sigma(nB), expOm(K), Om(K);
+  std::vector<bool> relTerm(K);
+  for (int i = 0; i < nB; ++i) out[(size_t)i * cols] = B[i];
+  for (int l = 0; l < niter; ++l) {
+    for (int k = 0; k < K; ++k) {
+      Sigma[0] = BF[0] - B[0] * Ei[k];
+      double mSigma = Sigma[0];
+      for (int i = 1; i < nB; ++i) {
+        Sigma[i] = BF[i] - B[i] * Ei[k];
+        if (mSigma < Sigma[i]) mSigma = Sigma[i];
+      }
+      double sD = 0;
+      for (int i = 0; i < nB; ++i) sD += R[i] * exp(Sigma[i] - mSigma);
+      if ((hist[k] == 0) || (sD == 0)) {
+        relTerm[k] = false;
+        lnOmega[k] = 0;
+      } else {
+        relTerm[k] = true;
+        lnOmega[k] = log(hist[k]) - mSigma - log(sD);
+      }
+    }
+    for (int i = 0; i < nB; ++i) {
+      double mOm = lnOmega[0] - B[i] * Ei[0];
+      for (int k = 0; k < K; ++k) {
+        Om[k] = lnOmega[k] - B[i] * Ei[k];
+        if (mOm < Om[k]) mOm = Om[k];
+      }
+      double pp = 0;
+      for (int k = 0; k < K; ++k) {
+        expOm[k] = exp(Om[k] - mOm);
+        if (relTerm[k]) pp += expOm[k];
+      }
+      double s = 0;
+      for (int k = 0; k < K; ++k)
+        if (relTerm[k]) s += Ei[k] * expOm[k];
+      double E_MHR = s / pp / N;
+      double BF_MHR = -mOm - log(pp);
+      BF[i] = BF_MHR;
+      s = 0;
+      for (int k = 0; k < K; ++k)
+        if (relTerm[k]) s += pow(Ei[k] - E_MHR * N, 2) * expOm[k];
+      double C_MHR = B[i] * B[i] * s / pp / N;
+      double* o = out + (size_t)i * cols + 1 + 3 * l;
+      o[0] = E_MHR;
+      o[1] = C_MHR;
+      o[2] = BF_MHR / N;
+    }
+  }
+}
+
+struct MhrOut {
+  int32_t* hist = nullptr;
+  double* mhr = nullptr;
+};
+
 /* ---------------- driver: main() of SSC:381-806 / MSC:546-957 ---------------- */
 struct Thermo {
   std::vector<double> B, Binc, E, C, M, M2, M4, lnQ, BF, S;
@@ -765,7 +834,8 @@ void fill_row(ora_row& row, const Thermo& th, int i, int N) {
 }
 
 template <typename W>
-int run_msc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, double* tot) {
+int run_msc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, double* tot,
+            const MhrOut& mo) {
   const int w = (int)sizeof(W) * 8;
   Geo g(p->L);
   const int N = g.N;
@@ -785,6 +855,8 @@ int run_msc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
     msc_init<W>(g, rg, th.R[0], p->seed, IS, words.data(), active.data());
     IS += (uint64_t)rg * EQT;
     msc_energy<W>(g, rg, words.data(), IE.data(), M.data());
+    std::vector<int32_t> H(p->mhr ? N + 1 : 0, 0);
+    if (p->mhr) hist_add(N, (int64_t)rg * w, IE.data(), active.data(), H);  // MSC:660-662
     ora_row* out = rows + (size_t)r * max_rows;
     int i = 1, iprev = 0;
     double deltaBeta = p->dB;
@@ -817,6 +889,7 @@ int run_msc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
       msc_sweep<W>(g, rg, words.data(), p->eqsweeps, p->seed, IS, T);
       IS += (uint64_t)rg * EQT;
       msc_energy<W>(g, rg, words.data(), IE.data(), M.data());
+      if (p->mhr) hist_add(N, (int64_t)rg * w, IE.data(), active.data(), H);  // MSC:759-762
       double Av[5];
       msc_averages(w, rg, active.data(), IE.data(), M.data(), Av);
       th.E[i] = Emean = Av[0] / th.R[i];
@@ -876,12 +949,26 @@ int run_msc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
     }
     // row 0 prints the never-assigned stack values as 0 (SSC:430-436, fixtures line 1)
     out[0].E = out[0].C = out[0].M = out[0].M2 = out[0].M4 = out[0].S = out[0].lnQ = 0.0;
+    if (p->mhr) {
+      // after the results file (SSC:695-791); th.BF[0] keeps the reweighted value for the
+      // next run, which prints and uses it
+      const int niter = p->mhr_iter > 0 ? p->mhr_iter : 1;
+      const int cols = 1 + 3 * niter;
+      std::vector<double> mo_rows((size_t)nB * cols);
+      mhr_analysis(N, nB, th.B.data(), th.BF.data(), th.R.data(), H.data(), niter,
+                   mo_rows.data());
+      if (mo.hist) memcpy(mo.hist + (size_t)r * (N + 1), H.data(), (size_t)(N + 1) * 4);
+      if (mo.mhr)
+        for (int k = 0; k < nB && k < max_rows; ++k)
+          memcpy(mo.mhr + ((size_t)r * max_rows + k) * cols, &mo_rows[(size_t)k * cols], cols * 8);
+    }
   }
   *tot = totPop;
   return 0;
 }
 
-int run_ssc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, double* tot) {
+int run_ssc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, double* tot,
+            const MhrOut& mo) {
   Geo g(p->L);
   const int N = g.N;
   Thermo th;
@@ -898,6 +985,8 @@ int run_ssc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
     ssc_init(g, th.R[0], p->seed, IS, p->umap, spins.data());
     IS += (uint64_t)th.R[0] * EQT;
     ssc_energy(g, th.R[0], spins.data(), IE.data(), M.data());
+    std::vector<int32_t> H(p->mhr ? N + 1 : 0, 0);
+    if (p->mhr) hist_add(N, th.R[0], IE.data(), nullptr, H);  // SSC:496-501
     ora_row* out = rows + (size_t)r * max_rows;
     int i = 1, iprev = 0;
     double deltaBeta = p->dB;
@@ -925,6 +1014,7 @@ int run_ssc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
       ssc_sweep(g, th.R[i], spins.data(), p->eqsweeps, p->seed, IS, T, p->umap);
       IS += (uint64_t)th.R[i] * EQT;
       ssc_energy(g, th.R[i], spins.data(), IE.data(), M.data());
+      if (p->mhr) hist_add(N, th.R[i], IE.data(), nullptr, H);  // SSC:608-611
       double Av[5];
       ssc_averages(th.R[i], IE.data(), M.data(), Av);
       th.E[i] = Emean = Av[0] / th.R[i];
@@ -981,6 +1071,19 @@ int run_ssc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
       }
     }
     out[0].E = out[0].C = out[0].M = out[0].M2 = out[0].M4 = out[0].S = out[0].lnQ = 0.0;
+    if (p->mhr) {
+      // after the results file (SSC:695-791); th.BF[0] keeps the reweighted value for the
+      // next run, which prints and uses it
+      const int niter = p->mhr_iter > 0 ? p->mhr_iter : 1;
+      const int cols = 1 + 3 * niter;
+      std::vector<double> mo_rows((size_t)nB * cols);
+      mhr_analysis(N, nB, th.B.data(), th.BF.data(), th.R.data(), H.data(), niter,
+                   mo_rows.data());
+      if (mo.hist) memcpy(mo.hist + (size_t)r * (N + 1), H.data(), (size_t)(N + 1) * 4);
+      if (mo.mhr)
+        for (int k = 0; k < nB && k < max_rows; ++k)
+          memcpy(mo.mhr + ((size_t)r * max_rows + k) * cols, &mo_rows[(size_t)k * cols], cols * 8);
+    }
   }
   *tot = totPop;
   return 0;
@@ -993,6 +1096,19 @@ extern "C" {
 
 int ora_run(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, double* elapsed_s,
             double* total_pop) {
+  return ora_run_ex(p, rows, max_rows, nrows, elapsed_s, total_pop, nullptr, nullptr);
+}
+
+void ora_mhr_analysis(int N, int nB, const double* B, double* BF, const int32_t* R,
+                      const int32_t* hist, int niter, double* out) {
+  mhr_analysis(N, nB, B, BF, R, hist, niter, out);
+}
+
+int ora_run_ex(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows,
+               double* elapsed_s, double* total_pop, int32_t* hist, double* mhr) {
+  MhrOut mo;
+  mo.hist = hist;
+  mo.mhr = mhr;
   if (!p || p->L < 2 || (p->L & 1) || p->Rinit < 1 || p->runs < 1) return -1;
   if (p->coding == ORA_MSC && p->msbits != 32 && p->msbits != 64) return -1;
   ora_params pp = *p;
@@ -1006,9 +1122,9 @@ int ora_run(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
   auto t0 = std::chrono::steady_clock::now();
   double tot = 0;
   int rc;
-  if (p->coding == ORA_SSC) rc = run_ssc(p, rows, max_rows, nrows, &tot);
-  else if (p->msbits == 32) rc = run_msc<uint32_t>(p, rows, max_rows, nrows, &tot);
-  else rc = run_msc<uint64_t>(p, rows, max_rows, nrows, &tot);
+  if (p->coding == ORA_SSC) rc = run_ssc(p, rows, max_rows, nrows, &tot, mo);
+  else if (p->msbits == 32) rc = run_msc<uint32_t>(p, rows, max_rows, nrows, &tot, mo);
+  else rc = run_msc<uint64_t>(p, rows, max_rows, nrows, &tot, mo);
   auto t1 = std::chrono::steady_clock::now();
   if (elapsed_s) *elapsed_s = std::chrono::duration<double>(t1 - t0).count();
   if (total_pop) *total_pop = tot;

@@ -54,6 +54,8 @@ typedef struct {
   int32_t pad0;
   double min_overlap;   /* MinOverlap (SSC:41)                                */
   double max_overlap;   /* MaxOverlap (SSC:42)                                */
+  int32_t mhr;          /* 1 = energy histograms + MHR analysis (#define MHR, SSC:22) */
+  int32_t mhr_iter;     /* MHR_Niter (SSC:53), 0 = 1                          */
 } ora_params;
 
 /* One row of PA_results_run_r.dat (SSC:679-690) plus the per-step internals the
@@ -71,6 +73,16 @@ typedef struct {
   uint64_t spin_hash;   /* hash of the spin configuration after the sweeps    */
   uint64_t IS_after;    /* Philox subsequence counter after the step          */
 } ora_row;
+
+/* ora_run plus the MHR build's outputs: hist (runs * (N+1) ints) receives each run's energy
+ * histogram, mhr (runs * max_rows * (1 + 3 mhr_iter) doubles) each run's
+ * PA_MHR_results_run_r.dat rows (B, then E/N, C/N, BF/N per iteration).  Either may be
+ * null. */
+int ora_run_ex(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows,
+               double* elapsed_s, double* total_pop, int32_t* hist, double* mhr);
+/* The MHR analysis of SSC:705-764 on given arrays (BF updated in place). */
+void ora_mhr_analysis(int N, int nB, const double* B, double* BF, const int32_t* R,
+                      const int32_t* hist, int niter, double* out);
 
 /* Full PA run(s).  rows has room for runs*max_rows rows; run r uses rows
  * [r*max_rows, r*max_rows + nrows[r]).  Returns 0, or 1 when the step count

@@ -5,6 +5,7 @@ in-tree shared library with ctypes and converts arguments.  There is no Python o
 fallback: if the library is missing or no GPU is present, calls raise.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -34,6 +35,7 @@ class Params(ctypes.Structure):
         ("threads", ctypes.c_int32), ("balance", ctypes.c_int32), ("adaptive", ctypes.c_int32),
         ("schedule_len", ctypes.c_int32), ("min_overlap", ctypes.c_double),
         ("max_overlap", ctypes.c_double), ("schedule", ctypes.POINTER(ctypes.c_double)),
+        ("mhr", ctypes.c_int32), ("pad1", ctypes.c_int32),
     ]
 
 
@@ -72,6 +74,9 @@ SIGNATURES = {
     "pa_run_set_graph": (_I, [_P, _I]),
     "pa_run_destroy": (_I, [_P]),
     "pa_run_all": (_I, [_P, _P, _I, _P, _P]),
+    "pa_run_all_hist": (_I, [_P, _P, _I, _P, _P, _P, _I]),
+    "pa_run_histogram": (_I, [_P, _P, _I]),
+    "pa_mhr_analysis": (_I, [_I, _I, _P, _P, _P, _P, _I, _P]),
     "pa_batch_create": (_I, [_P, _I, _I, _PP]),
     "pa_batch_run": (_I, [_P]),
     "pa_batch_results": (_I, [_P, _I, _P, _I, _P]),
@@ -121,12 +126,12 @@ def _ptr(a):
 def make_params(L=64, coding=MSC, msbits=32, Rinit=20000, eqsweeps=100, dB=0.005, Binit=0.0,
                 Bfin=1.0, seed=100, initial_sequence=0, uniform=UNIFORM_CUDA101, max_steps=0,
                 trace=0, slice_bits=0, threads=0, balance=0, adaptive=0, min_overlap=0.0,
-                max_overlap=0.0, schedule=None):
+                max_overlap=0.0, schedule=None, mhr=0):
     """pa_params.  adaptive=1: the reference's adaptive beta step; schedule: an explicit
     list of betas (Binit first) to replay, e.g. run 0's adaptive steps for later runs."""
     p = Params(L, coding, msbits, Rinit, eqsweeps, dB, Binit, Bfin, seed & (2**64 - 1),
                initial_sequence, uniform, max_steps, trace, slice_bits, threads, balance,
-               adaptive, 0, min_overlap, max_overlap, None)
+               adaptive, 0, min_overlap, max_overlap, None, mhr, 0)
     if schedule is not None:
         arr = (ctypes.c_double * len(schedule))(*schedule)
         p._schedule_keep = arr  # the library copies it at creation; keep it alive until then
@@ -178,6 +183,43 @@ def run_all(params, max_rows=10000):
     fin = ctypes.c_uint64(0)
     _check(lib().pa_run_all(ctypes.byref(params), buf, max_rows, ctypes.byref(n), ctypes.byref(fin)))
     return _rows(buf, n.value), fin.value
+
+
+def run_all_hist(params, max_rows=10000):
+    """run_all of a params.mhr = 1 run; returns (rows, final counter, energy histogram)."""
+    buf = (Row * max_rows)()
+    n = ctypes.c_int(0)
+    fin = ctypes.c_uint64(0)
+    nb = params.L * params.L + 1
+    hist = (ctypes.c_int32 * nb)()
+    _check(lib().pa_run_all_hist(ctypes.byref(params), buf, max_rows, ctypes.byref(n),
+                                 ctypes.byref(fin), hist, nb))
+    return _rows(buf, n.value), fin.value, list(hist)
+
+
+def mhr_analysis(N, B, BF, R, hist, niter=1):
+    """Multi-histogram reweighting (PAisingSSC.cu:695-764).  B, BF (unnormalised), R per row;
+    returns (rows of [B, (E/N, C/N, BF/N) per iteration], BF after the last iteration)."""
+    nB = len(B)
+    Ba = (ctypes.c_double * nB)(*B)
+    BFa = (ctypes.c_double * nB)(*BF)
+    Ra = (ctypes.c_int32 * nB)(*R)
+    Ha = (ctypes.c_int32 * (N + 1))(*hist)
+    cols = 1 + 3 * niter
+    out = (ctypes.c_double * (nB * cols))()
+    _check(lib().pa_mhr_analysis(N, nB, Ba, BFa, Ra, Ha, niter, out))
+    return [list(out[i * cols:(i + 1) * cols]) for i in range(nB)], list(BFa)
+
+
+def unnormalised_bf(rows, N, bf0=None):
+    """BF_i = -N ln 2 - sum_{j<=i} lnQ_j (PAisingSSC.cu:435,627-628) in the reference's order;
+    bf0 replaces BF_0 (the reference keeps BF[0] across -M runs, where MHR overwrites it)."""
+    out = [-N * math.log(2.0) if bf0 is None else bf0]
+    s = 0.0
+    for r in rows[1:]:
+        s -= r.lnQ
+        out.append(-N * math.log(2.0) + s)
+    return out
 
 
 class Run:

@@ -8,6 +8,8 @@
 //   -u conv   uniform-double convention: 0 = CUDA >= 10.1 (default), 1 = CUDA <= 10.0
 //   -a        adaptive inverse-temperature step (the reference's AdaptiveStep build,
 //             PAisingSSC.cu:23,40-43,406-410,644-660); later -M runs replay run 0's steps
+//   -H        multi-histogram reweighting (the reference's MHR build, PAisingSSC.cu:22,52-54,
+//             695-791): PA_MHR_results_run_<r>.dat with MHR_Niter = 1
 // The GPU work goes through the C ABI of paising.h (pa_run_all per run).
 #include <sys/stat.h>
 
@@ -84,7 +86,8 @@ void usage(const char* prog) {
             << " -w bits            ( bits            = replicas per multi-spin word, 32 or 64 )\n"
 #endif
             << " -u convention      ( convention      = uniform-double mapping: 0 CUDA>=10.1, 1 CUDA<=10.0 )\n"
-            << " -a                 ( adaptive inverse temperature step, MinOverlap 0.85, MaxOverlap 0.87 )\n";
+            << " -a                 ( adaptive inverse temperature step, MinOverlap 0.85, MaxOverlap 0.87 )\n"
+            << " -H                 ( multi-histogram reweighting analysis, PA_MHR_results_run_<r>.dat )\n";
 }
 
 }  // namespace
@@ -97,10 +100,10 @@ int main(int argc, char** argv) {
   long long seed_opt = (long long)time(NULL);
   unsigned int EQsweeps = 100;
   double Bfin = 1.0, dBinit = 0.005;
-  int Rinit = 20000, runs = 1, OutputPrecision = 11, L = 64, bits = 32, uconv = 0, adaptive = 0;
+  int Rinit = 20000, runs = 1, OutputPrecision = 11, L = 64, bits = 32, uconv = 0, adaptive = 0, mhr = 0;
   int optc, opti;
   double optf;
-  while ((optc = get_opt(argc, argv, msc ? "R:t:d:f:M:s:P:o:L:w:u:a?" : "R:t:d:f:M:s:P:o:L:u:a?")) != -1) {
+  while ((optc = get_opt(argc, argv, msc ? "R:t:d:f:M:s:P:o:L:w:u:aH?" : "R:t:d:f:M:s:P:o:L:u:aH?")) != -1) {
     switch (optc) {
       case 'R': opti = atoi(opt_arg); if (opti) Rinit = opti; break;
       case 't': opti = atoi(opt_arg); EQsweeps = opti; break;
@@ -118,6 +121,7 @@ int main(int argc, char** argv) {
       case 'w': opti = atoi(opt_arg); if (opti) bits = opti; break;
       case 'u': uconv = atoi(opt_arg); break;
       case 'a': adaptive = 1; break;
+      case 'H': mhr = 1; break;
       case '?': usage(prog); return 1;
     }
   }
@@ -182,6 +186,11 @@ int main(int argc, char** argv) {
   std::vector<pa_row> rows(10000);
   std::vector<double> sched;  // adaptive: run 0's betas, replayed by later runs (SSC:662)
   p.adaptive = adaptive;
+  p.mhr = mhr;
+  std::vector<int32_t> hist(mhr ? N + 1 : 0);
+  // MHR overwrites BF[0] (set once before the runs, SSC:435) and the next run prints and
+  // uses the overwritten value (SSC:678-690, 711)
+  double BF0 = -N * std::log(2.0);
   for (int r = 0; r < runs; ++r) {
     p.initial_sequence = IS;
     if (adaptive && r > 0) {
@@ -191,7 +200,9 @@ int main(int argc, char** argv) {
     }
     int nrows = 0;
     uint64_t is_end = 0;
-    const int rc = pa_run_all(&p, rows.data(), (int)rows.size(), &nrows, &is_end);
+    const int rc = mhr ? pa_run_all_hist(&p, rows.data(), (int)rows.size(), &nrows, &is_end,
+                                         hist.data(), N + 1)
+                       : pa_run_all(&p, rows.data(), (int)rows.size(), &nrows, &is_end);
     if (rc == PA_ERR_NBMAX) {
       if (adaptive)
         fprintf(stderr,
@@ -213,6 +224,7 @@ int main(int argc, char** argv) {
     if (adaptive && r == 0)
       for (int i = 0; i < nrows; ++i) sched.push_back(rows[i].B);
     for (int i = 1; i < nrows; ++i) totPop += rows[i].R;
+    if (mhr) rows[0].BF = BF0 / N;
     char str[700];
     snprintf(str, sizeof str, "%sPA_results_run_%d.dat", dataDir, r);
     std::ofstream results(str);
@@ -221,6 +233,35 @@ int main(int argc, char** argv) {
       const pa_row& w = rows[i];
       results << w.B << " " << w.E << " " << w.C << " " << w.M << " " << w.M2 << " " << w.M4 << " "
               << w.BF << " " << w.S << " " << w.R << " " << w.lnQ << "\n";
+    }
+    if (mhr) {
+      // BF_i = BF_0 + sum of -lnQ in the reference's order (SSC:627-628), R_i, B_i
+      std::vector<double> B(nrows), BF(nrows);
+      std::vector<int32_t> R(nrows);
+      double sumlnQ = 0.0;
+      for (int i = 0; i < nrows; ++i) {
+        B[i] = rows[i].B;
+        R[i] = rows[i].R;
+        if (i == 0) { BF[i] = BF0; continue; }
+        sumlnQ -= rows[i].lnQ;
+        BF[i] = -N * std::log(2.0) + sumlnQ;
+      }
+      const int niter = 1;  // MHR_Niter, SSC:53
+      std::vector<double> out((size_t)nrows * (1 + 3 * niter));
+      if (pa_mhr_analysis(N, nrows, B.data(), BF.data(), R.data(), hist.data(), niter, out.data())) {
+        fprintf(stderr, "%s\n", pa_last_error());
+        return 1;
+      }
+      BF0 = BF[0];
+      snprintf(str, sizeof str, "%sPA_MHR_results_run_%d.dat", dataDir, r);
+      std::ofstream mres(str);
+      mres.precision(OutputPrecision);
+      for (int i = 0; i < nrows; ++i) {
+        const double* o = &out[(size_t)i * (1 + 3 * niter)];
+        mres << o[0] << " ";
+        for (int l = 0; l < niter; ++l) mres << o[1 + 3 * l] << " " << o[2 + 3 * l] << " " << o[3 + 3 * l] << " ";
+        mres << "\n";
+      }
     }
   }
   const double Etime =

@@ -66,6 +66,7 @@ struct RunDesc {
   int32_t* par;             // parent of every child of the last resampling [cap_pop]
   uint32_t* flags;          // balanced sweeps: hand-over flag per (group, slice) [cap_units*8]
   double* gsum;             // MSC per-group lane sums of the q terms [cap_units]
+  int32_t* hist;            // MHR energy histogram [N + 1] (null unless requested)
   int32_t rs_sites;         // MSC resampling: site words per workgroup chunk
   int32_t rs_split;         // MSC resampling: workgroups sharing one site chunk (new groups split)
   uint32_t* cblk;           // per-1024 chunk sums of Roff

@@ -274,6 +274,7 @@ size_t layout(RunHost& h, uint8_t* base) {
   d.par = (int32_t*)take((size_t)h.cap_pop * 4);
   d.flags = (uint32_t*)take((size_t)h.cap_units * 8 * 4);
   d.gsum = (double*)take((size_t)h.cap_units * 8);
+  d.hist = h.p.mhr ? (int32_t*)take((size_t)(h.N + 1) * 4) : nullptr;
   d.cblk = (uint32_t*)take((size_t)h.blocks(kKChunks) * 4);
   d.qblk = (double*)take((size_t)h.blocks(kKGroupChunks) * 8 + 8 * (size_t)h.blocks(kKChunks));
   d.avblk = (double*)take((size_t)h.blocks(kKAvg) * 5 * 8);
@@ -330,6 +331,7 @@ int materialize(RunHost& h) {
   HIPCHECK(hipMemcpy((void*)d.thr, thr.data(), thr.size() * 8, hipMemcpyHostToDevice));
   HIPCHECK(hipMemset(d.out, 0, (h.n_steps + 1) * sizeof(StepOut)));
   HIPCHECK(hipMemset(d.flags, 0, (size_t)h.cap_units * 8 * 4));
+  if (d.hist) HIPCHECK(hipMemset(d.hist, 0, (size_t)(h.N + 1) * 4));
   RunState st{};
   st.step = 1;
   st.R = h.p.Rinit;
@@ -414,6 +416,7 @@ struct Batch {
   bool has_w[2] = {false, false};
   std::vector<Group> groups;
   bool any_trace = false;
+  bool any_mhr = false;
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -462,6 +465,7 @@ struct Batch {
     std::vector<int32_t> offs((size_t)kKindCount * (n + 1), 0);
     std::vector<uint64_t> is0(n);
     any_trace = false;
+    any_mhr = false;
     max_steps = 0;
     for (int k = 0; k < kKindCount; ++k) {
       int acc = 0;
@@ -483,6 +487,7 @@ struct Batch {
         rs_lds[wi] = std::max(rs_lds[wi], runs[r]->resample_lds());
       }
       any_trace |= runs[r]->p.trace != 0;
+      any_mhr |= runs[r]->p.mhr != 0;
       max_steps = std::max(max_steps, runs[r]->n_steps);
     }
     int rc;
@@ -551,6 +556,10 @@ struct Batch {
       HIPCHECK(pa_launch_init(g.v.coding, g.v.w, g.v.sw, g.v.nt, g.units_blocks, g.lds, stream,
                               g.desc_d, g.offs_d, (int)g.members.size(), init_is0[i]));
     }
+    // MHR: the initial population enters the histogram too (SSC:500)
+    if (any_mhr)
+      HIPCHECK(pa_launch_step(14, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d,
+                              (int)runs.size()));
     return PA_OK;
   }
 
@@ -612,6 +621,7 @@ struct Batch {
     if (time_sweep) HIPCHECK(hipEventRecord(t2, stream));
     HIPCHECK(pa_launch_step(5, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n));
     if (any_trace) HIPCHECK(pa_launch_step(6, 0, 0, 0, 0, blocks[kKUnits], 0, stream, desc_d, offs_d, n));
+    if (any_mhr) HIPCHECK(pa_launch_step(13, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(7, 0, 0, 0, 0, n, 0, stream, desc_d, offs_d, n));
     return PA_OK;
   }
@@ -1038,7 +1048,85 @@ int pa_run_destroy(pa_run* r) {
   return PA_OK;
 }
 
+int pa_run_histogram(pa_run* r, int32_t* hist, int nbins) {
+  if (!r || !hist) return fail(PA_ERR_INVALID, "null argument");
+  const RunHost& h = *r->host;
+  if (!h.desc.hist) return fail(PA_ERR_STATE, "run created without mhr = 1");
+  if (nbins < h.N + 1) return fail(PA_ERR_INVALID, "histogram needs N + 1 bins");
+  HIPCHECK(hipStreamSynchronize(r->batch.stream));
+  HIPCHECK(hipMemcpy(hist, h.desc.hist, (size_t)(h.N + 1) * 4, hipMemcpyDeviceToHost));
+  return PA_OK;
+}
+
+// MHR analysis, PAisingSSC.cu:705-764 (identical in PAisingMSC.cu:855-914).  Host FP64 with
+// the C library's exp, log and pow, in the reference's order of operations.
+int pa_mhr_analysis(int N, int nB, const double* B, double* BF, const int32_t* R,
+                    const int32_t* hist, int niter, double* out) {
+  if (N < 1 || nB < 1 || !B || !BF || !R || !hist || niter < 1 || !out)
+    return fail(PA_ERR_INVALID, "bad MHR arguments");
+  const int nk = N + 1;
+  std::vector<double> Ei(nk), lnOmega(nk), Sigma(nB), expOm(nk), Om(nk);
+  std::vector<char> rel(nk);
+  for (int k = 0; k < nk; ++k) Ei[k] = 4 * k - 2 * N;  // SSC:461-463
+  const int cols = 1 + 3 * niter;
+  for (int i = 0; i < nB; ++i) out[(size_t)i * cols] = B[i];
+  for (int l = 0; l < niter; ++l) {
+    // density of states from the summed histogram (SSC:707-731)
+    for (int k = 0; k < nk; ++k) {
+      double mS = BF[0] - B[0] * Ei[k];
+      Sigma[0] = mS;
+      for (int i = 1; i < nB; ++i) {
+        Sigma[i] = BF[i] - B[i] * Ei[k];
+        if (mS < Sigma[i]) mS = Sigma[i];
+      }
+      double sD = 0;
+      for (int i = 0; i < nB; ++i) sD += R[i] * std::exp(Sigma[i] - mS);
+      if (hist[k] == 0 || sD == 0) {
+        rel[k] = 0;
+        lnOmega[k] = 0;
+      } else {
+        rel[k] = 1;
+        lnOmega[k] = std::log((double)hist[k]) - mS - std::log(sD);
+      }
+    }
+    // reweighted observables per temperature (SSC:733-764); the maximum runs over every
+    // bin, relevant or not, as in the reference
+    for (int i = 0; i < nB; ++i) {
+      double mOm = lnOmega[0] - B[i] * Ei[0];
+      for (int k = 0; k < nk; ++k) {
+        Om[k] = lnOmega[k] - B[i] * Ei[k];
+        if (mOm < Om[k]) mOm = Om[k];
+      }
+      double p = 0;
+      for (int k = 0; k < nk; ++k) {
+        expOm[k] = std::exp(Om[k] - mOm);
+        if (rel[k]) p += expOm[k];
+      }
+      double s = 0;
+      for (int k = 0; k < nk; ++k)
+        if (rel[k]) s += Ei[k] * expOm[k];
+      const double E = s / p / N;
+      const double F = -mOm - std::log(p);
+      BF[i] = F;
+      s = 0;
+      for (int k = 0; k < nk; ++k)
+        if (rel[k]) s += std::pow(Ei[k] - E * N, 2) * expOm[k];
+      const double C = B[i] * B[i] * s / p / N;
+      double* o = out + (size_t)i * cols + 1 + 3 * l;
+      o[0] = E;
+      o[1] = C;
+      o[2] = F / N;
+    }
+  }
+  return PA_OK;
+}
+
 int pa_run_all(const pa_params* p, pa_row* rows, int max_rows, int* nrows, uint64_t* final_is) {
+  return pa_run_all_hist(p, rows, max_rows, nrows, final_is, nullptr, 0);
+}
+
+int pa_run_all_hist(const pa_params* p, pa_row* rows, int max_rows, int* nrows,
+                    uint64_t* final_is, int32_t* hist, int nbins) {
   double sigma = 10.0;
   for (int attempt = 0; attempt < 4; ++attempt, sigma *= 4.0) {
     pa_run* r = nullptr;
@@ -1049,6 +1137,7 @@ int pa_run_all(const pa_params* p, pa_row* rows, int max_rows, int* nrows, uint6
     if (rc == PA_ERR_CAPACITY) { pa_run_destroy(r); continue; }
     if (!rc) rc = rows_of(*r->host, rows, max_rows, nrows);
     if (!rc && final_is) *final_is = r->host->st_h.IS;
+    if (!rc && hist) rc = pa_run_histogram(r, hist, nbins);
     pa_run_destroy(r);
     return rc;
   }

@@ -532,6 +532,29 @@ __global__ __launch_bounds__(1024) void k_overlap(const RunDesc* __restrict__ ru
 }
 
 // ============================================================================================
+// k_hist: adds the population's energies to the run's MHR histogram, bin (2N + E) / 4
+// (UpdateShistE, SSC:296-302 / MSC:454-460).  init = 1: the initial population (buffer
+// st->buf, R members); otherwise the population just equilibrated by this step (buffer
+// st->buf ^ 1, Rnew members; launched before k_fin).  Integer atomics: order-free.
+// ============================================================================================
+__global__ __launch_bounds__(1024) void k_hist(const RunDesc* __restrict__ runs,
+                                               const int32_t* __restrict__ offs, int nruns,
+                                               int init) {
+  const int* off = offs + kKChunks * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!d.hist || st->overflow || (!init && !st->active)) return;
+  const int c = blockIdx.x - off[r];
+  const int R = init ? st->R : st->Rnew, buf = init ? st->buf : st->buf ^ 1, w = d.w;
+  const int n = c * kRB + threadIdx.x;
+  bool in;
+  if (d.coding == kSSC) in = n < R;
+  else in = n / w < ceil_div(R, w) && ((d.amask[buf][n / w] >> (n % w)) & 1);
+  if (in) atomicAdd(&d.hist[(2 * d.N + d.IE[n]) / 4], 1);
+}
+
+// ============================================================================================
 // k_scan: exclusive prefix of Roff in population order (CalcParSum), R_i, capacity check.
 // ============================================================================================
 __global__ __launch_bounds__(1024) void k_scan(const RunDesc* __restrict__ runs,
@@ -1357,6 +1380,8 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 9: hipLaunchKernelGGL(k_parents, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 11: return launch_sweep_msc_bal(w, sw, nt, nblocks, lds, s, runs, nruns);
     case 12: hipLaunchKernelGGL(k_overlap, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
+    case 13: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 0); break;
+    case 14: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 1); break;
     case 10:
       if (w == 32) hipLaunchKernelGGL(k_resample_msc<uint32_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
       else hipLaunchKernelGGL(k_resample_msc<uint64_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
