@@ -10,6 +10,10 @@
 //             PAisingSSC.cu:23,40-43,406-410,644-660); later -M runs replay run 0's steps
 //   -H        multi-histogram reweighting (the reference's MHR build, PAisingSSC.cu:22,52-54,
 //             695-791): PA_MHR_results_run_<r>.dat with MHR_Niter = 1
+//   -e        PA_energies_<i>.dat after every step (EnergiesPopStore, PAisingSSC.cu:24,
+//             563-578; PAisingMSC.cu:743-757)
+//   -l        PA_lattice_<i>.dat after every step of run 0 (LatticePopStore, SSC only,
+//             PAisingSSC.cu:25,424-427,580-605)
 // The GPU work goes through the C ABI of paising.h (pa_run_all per run).
 #include <sys/stat.h>
 
@@ -87,7 +91,91 @@ void usage(const char* prog) {
 #endif
             << " -u convention      ( convention      = uniform-double mapping: 0 CUDA>=10.1, 1 CUDA<=10.0 )\n"
             << " -a                 ( adaptive inverse temperature step, MinOverlap 0.85, MaxOverlap 0.87 )\n"
-            << " -H                 ( multi-histogram reweighting analysis, PA_MHR_results_run_<r>.dat )\n";
+            << " -H                 ( multi-histogram reweighting analysis, PA_MHR_results_run_<r>.dat )\n"
+            << " -e                 ( store the population's energies at each temperature, PA_energies_<i>.dat )\n"
+#if PA_CLI_CODING == 0
+            << " -l                 ( store the population's lattices at each temperature of run 0, PA_lattice_<i>.dat )\n"
+#endif
+            ;
+}
+
+// One run stepped on the device one beta step at a time, writing the population dumps after
+// every step (the reference writes them after the energies, before the averages).
+int run_with_dumps(const pa_params& p, const char* dataDir, int prec, bool store_e, bool store_l,
+                   std::vector<pa_row>& rows, int* nrows, uint64_t* is_end, int32_t* hist) {
+  pa_run* run = nullptr;
+  int rc = pa_run_create(&p, &run);
+  if (rc) return rc;
+  const int L = p.L, N = L * L, Nh = N / 2, Ld2 = L / 2;
+  std::vector<int32_t> IE;
+  std::vector<uint64_t> mask;
+  std::vector<int8_t> pop;
+  int last = 0;
+  for (;;) {
+    if ((rc = pa_run_step(run, 1))) break;
+    int done = 0, total = 0;
+    if ((rc = pa_run_steps_done(run, &done, &total))) break;
+    if (done == last) break;  // schedule finished
+    last = done;
+    char path[700];
+    if (store_e) {
+      size_t bytes = 0;
+      if ((rc = pa_run_array(run, 0, nullptr, &bytes))) break;
+      IE.resize(bytes / 4);
+      if ((rc = pa_run_array(run, 0, IE.data(), &bytes))) break;
+      snprintf(path, sizeof path, "%sPA_energies_%d.dat", dataDir, done);
+      std::ofstream f(path);
+      f.precision(prec);
+      if (p.coding == PA_SSC) {
+        for (size_t j = 0; j < IE.size(); ++j) f << IE[j] << " ";
+      } else {  // active lanes, group-major (PAisingMSC.cu:754-756)
+        if ((rc = pa_run_array(run, 4, nullptr, &bytes))) break;
+        mask.resize(bytes / 8);
+        if ((rc = pa_run_array(run, 4, mask.data(), &bytes))) break;
+        for (size_t g = 0; g < mask.size(); ++g)
+          for (int l = 0; l < p.msbits; ++l)
+            if ((mask[g] >> l) & 1) f << IE[g * p.msbits + l] << " ";
+      }
+    }
+    if (store_l) {
+      size_t bytes = 0;
+      if ((rc = pa_run_population(run, nullptr, &bytes))) break;
+      pop.resize(bytes);
+      if ((rc = pa_run_population(run, pop.data(), &bytes))) break;
+      pa_row cur[1];
+      int nr = 0;
+      std::vector<pa_row> tmp(done + 1);
+      if ((rc = pa_run_results(run, tmp.data(), done + 1, &nr))) break;
+      cur[0] = tmp[done];
+      snprintf(path, sizeof path, "%sPA_lattice_%d.dat", dataDir, done);
+      std::ofstream f(path, std::ofstream::out | std::ofstream::app);
+      f.precision(prec);
+      f << "i=" << done << "; beta=" << cur[0].B << "; R=" << cur[0].R << ";" << std::endl;
+      const int R = cur[0].R;
+      int ABflag = 0;
+      for (int j = 0; j < R; ++j) {
+        const int8_t* gA = &pop[(size_t)j * N];
+        const int8_t* gB = gA + Nh;
+        f << "P={";
+        int k = 0;
+        while (k < Nh) {
+          if ((k % Ld2) == 0) { f << "{"; ABflag = 1 - ABflag; }
+          if (ABflag) f << (int)gA[k] << "," << (int)gB[k];
+          else f << (int)gB[k] << "," << (int)gA[k];
+          k++;
+          if ((k % Ld2) == 0) { f << "}"; if (k < Nh) f << ","; }
+          else f << ",";
+        }
+        f << "};" << std::endl;
+      }
+      f << std::endl;
+    }
+  }
+  if (!rc) rc = pa_run_results(run, rows.data(), (int)rows.size(), nrows);
+  if (!rc) rc = pa_run_final_sequence(run, is_end);
+  if (!rc && hist) rc = pa_run_histogram(run, hist, N + 1);
+  pa_run_destroy(run);
+  return rc;
 }
 
 }  // namespace
@@ -100,10 +188,10 @@ int main(int argc, char** argv) {
   long long seed_opt = (long long)time(NULL);
   unsigned int EQsweeps = 100;
   double Bfin = 1.0, dBinit = 0.005;
-  int Rinit = 20000, runs = 1, OutputPrecision = 11, L = 64, bits = 32, uconv = 0, adaptive = 0, mhr = 0;
+  int Rinit = 20000, runs = 1, OutputPrecision = 11, L = 64, bits = 32, uconv = 0, adaptive = 0, mhr = 0, store_e = 0, store_l = 0;
   int optc, opti;
   double optf;
-  while ((optc = get_opt(argc, argv, msc ? "R:t:d:f:M:s:P:o:L:w:u:aH?" : "R:t:d:f:M:s:P:o:L:u:aH?")) != -1) {
+  while ((optc = get_opt(argc, argv, msc ? "R:t:d:f:M:s:P:o:L:w:u:aHe?" : "R:t:d:f:M:s:P:o:L:u:aHel?")) != -1) {
     switch (optc) {
       case 'R': opti = atoi(opt_arg); if (opti) Rinit = opti; break;
       case 't': opti = atoi(opt_arg); EQsweeps = opti; break;
@@ -122,6 +210,8 @@ int main(int argc, char** argv) {
       case 'u': uconv = atoi(opt_arg); break;
       case 'a': adaptive = 1; break;
       case 'H': mhr = 1; break;
+      case 'e': store_e = 1; break;
+      case 'l': store_l = 1; break;
       case '?': usage(prog); return 1;
     }
   }
@@ -200,9 +290,14 @@ int main(int argc, char** argv) {
     }
     int nrows = 0;
     uint64_t is_end = 0;
-    const int rc = mhr ? pa_run_all_hist(&p, rows.data(), (int)rows.size(), &nrows, &is_end,
-                                         hist.data(), N + 1)
-                       : pa_run_all(&p, rows.data(), (int)rows.size(), &nrows, &is_end);
+    int rc;
+    if (store_e || (store_l && r == 0))
+      rc = run_with_dumps(p, dataDir, OutputPrecision, store_e, store_l && r == 0, rows, &nrows,
+                          &is_end, mhr ? hist.data() : nullptr);
+    else if (mhr)
+      rc = pa_run_all_hist(&p, rows.data(), (int)rows.size(), &nrows, &is_end, hist.data(), N + 1);
+    else
+      rc = pa_run_all(&p, rows.data(), (int)rows.size(), &nrows, &is_end);
     if (rc == PA_ERR_NBMAX) {
       if (adaptive)
         fprintf(stderr,
