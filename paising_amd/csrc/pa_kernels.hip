@@ -173,6 +173,43 @@ __device__ __forceinline__ void lcg_cmp2(uint32_t& a4, uint32_t& a8, uint32_t r,
       : [t4] "s"(T4), [t8] "s"(T8), [r] "v"(r));
 }
 
+// Four chains at once, ordered so that every mask is written at least four VALU
+// instructions before its carry-in read (no s_nop needed) with four mask pairs and one
+// shared dummy carry-out; then the four LCG steps.
+__device__ __forceinline__ void lcg_cmp_step4(uint32_t (&a4)[4], uint32_t (&a8)[4],
+                                              uint32_t (&r)[4], uint32_t T4, uint32_t T8,
+                                              uint64_t c64) {
+  uint64_t m0, m1, m2, m3, cc, p[4];
+  asm("v_cmp_gt_u32_e64 %[m0], %[t4], %[r0]\n\t"
+      "v_cmp_gt_u32_e64 %[m1], %[t8], %[r0]\n\t"
+      "v_cmp_gt_u32_e64 %[m2], %[t4], %[r1]\n\t"
+      "v_cmp_gt_u32_e64 %[m3], %[t8], %[r1]\n\t"
+      "v_addc_co_u32_e64 %[a40], %[cc], %[a40], %[a40], %[m0]\n\t"
+      "v_addc_co_u32_e64 %[a80], %[cc], %[a80], %[a80], %[m1]\n\t"
+      "v_cmp_gt_u32_e64 %[m0], %[t4], %[r2]\n\t"
+      "v_cmp_gt_u32_e64 %[m1], %[t8], %[r2]\n\t"
+      "v_addc_co_u32_e64 %[a41], %[cc], %[a41], %[a41], %[m2]\n\t"
+      "v_addc_co_u32_e64 %[a81], %[cc], %[a81], %[a81], %[m3]\n\t"
+      "v_cmp_gt_u32_e64 %[m2], %[t4], %[r3]\n\t"
+      "v_cmp_gt_u32_e64 %[m3], %[t8], %[r3]\n\t"
+      "v_addc_co_u32_e64 %[a42], %[cc], %[a42], %[a42], %[m0]\n\t"
+      "v_addc_co_u32_e64 %[a82], %[cc], %[a82], %[a82], %[m1]\n\t"
+      "v_addc_co_u32_e64 %[a43], %[cc], %[a43], %[a43], %[m2]\n\t"
+      "v_addc_co_u32_e64 %[a83], %[cc], %[a83], %[a83], %[m3]\n\t"
+      "v_mad_u64_u32 %[p0], %[cc], %[A], %[r0], %[C]\n\t"
+      "v_mad_u64_u32 %[p1], %[cc], %[A], %[r1], %[C]\n\t"
+      "v_mad_u64_u32 %[p2], %[cc], %[A], %[r2], %[C]\n\t"
+      "v_mad_u64_u32 %[p3], %[cc], %[A], %[r3], %[C]"
+      : [a40] "+v"(a4[0]), [a80] "+v"(a8[0]), [a41] "+v"(a4[1]), [a81] "+v"(a8[1]),
+        [a42] "+v"(a4[2]), [a82] "+v"(a8[2]), [a43] "+v"(a4[3]), [a83] "+v"(a8[3]),
+        [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [cc] "=&s"(cc),
+        [p0] "=&v"(p[0]), [p1] "=&v"(p[1]), [p2] "=&v"(p[2]), [p3] "=&v"(p[3])
+      : [t4] "s"(T4), [t8] "s"(T8), [r0] "v"(r[0]), [r1] "v"(r[1]), [r2] "v"(r[2]),
+        [r3] "v"(r[3]), [A] "s"(kLcgA), [C] "v"(c64));
+#pragma unroll
+  for (int j = 0; j < 4; ++j) r[j] = (uint32_t)p[j];
+}
+
 // Condition masks for SW consecutive LCG values starting at `ran` (NW independent chains
 // interleaved for instruction-level parallelism): bit i of c4 (c8) is [ran_i < T4]
 // ([ran_i < T8]).  The masks are accumulated most-significant-first and bit-reversed.
@@ -191,10 +228,14 @@ __device__ __forceinline__ void lcg_masks(uint32_t (&ran)[NW], uint32_t T4, uint
     for (int j = 0; j < NW; ++j) { a4[j] = 0; a8[j] = 0; }
 #pragma unroll
     for (int i = 0; i < HB; ++i) {
+      if constexpr (NW == 4) {
+        lcg_cmp_step4(a4, a8, ran, T4, T8, c64);
+      } else {
 #pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        lcg_cmp2(a4[j], a8[j], ran[j], T4, T8);
-        ran[j] = lcg_next(ran[j], c64);
+        for (int j = 0; j < NW; ++j) {
+          lcg_cmp2(a4[j], a8[j], ran[j], T4, T8);
+          ran[j] = lcg_next(ran[j], c64);
+        }
       }
     }
 #pragma unroll
