@@ -85,9 +85,9 @@ uint64_t ssc_threshold(double T, int umap) {
 }
 
 struct SweepVariant {
-  int coding, w, sw, nt;
+  int coding, w, sw, nt, path;  // path: SSC sweep kernel (1 = register-cached 3-block loop)
   bool operator<(const SweepVariant& o) const {
-    return std::tie(coding, w, sw, nt) < std::tie(o.coding, o.w, o.sw, o.nt);
+    return std::tie(coding, w, sw, nt, path) < std::tie(o.coding, o.w, o.sw, o.nt, o.path);
   }
 };
 
@@ -95,7 +95,7 @@ struct RunHost {
   pa_params p{};
   int w = 1, N = 0, n_steps = 0, cap_pop = 0, cap_units = 0;
   int R0 = 0;
-  int sw = 0, nt = 0;
+  int sw = 0, nt = 0, path = 0;
   int rs_sites = 1;
   int rs_split = 1;
   int rs_lp = 1;
@@ -228,6 +228,15 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma) {
   if (h.nt != 128 && h.nt != 256 && h.nt != 512)
     return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512");
   if (!aligned) h.nt = 128;
+  // SSC sweep kernel path (pa_kernels.hip, ssc_sweeps): register-cached block loop for two
+  // or three Philox blocks per thread and half sweep, register-cached stream walk for
+  // unaligned lattices with at most 4 / 9 sites per stream and half sweep
+  if (p.coding == PA_SSC) {
+    const int ntmax = ceil_div(Nh, 128);
+    if (aligned && nb > h.nt / 128 && nb <= 3 * (h.nt / 128)) h.path = 1;
+    else if (!aligned && ntmax <= 4) h.path = 2;
+    else if (!aligned && ntmax <= 9) h.path = 3;
+  }
   if (p.coding == PA_MSC) {
     int sw = p.slice_bits;
     if (sw == 0) {
@@ -503,7 +512,7 @@ struct Batch {
       std::map<SweepVariant, std::vector<int>> gm;
       for (int r = 0; r < n; ++r) {
         const RunHost& h = *runs[r];
-        SweepVariant v{h.p.coding, h.w, h.sw, h.nt};
+        SweepVariant v{h.p.coding, h.w, h.sw, h.nt, h.path};
         gm[v].push_back(r);
       }
       for (auto& kv : gm) {
@@ -615,7 +624,8 @@ struct Batch {
         HIPCHECK(pa_launch_step(11, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, stream,
                                 g.desc_d, g.offs_d, (int)g.members.size()));
       else
-        HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.units_blocks * ns, g.lds, stream,
+        HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.coding == PA_SSC ? g.v.path : g.v.sw, g.v.nt,
+                                g.units_blocks * ns, g.lds, stream,
                                 g.desc_d, g.offs_d, (int)g.members.size()));
     }
     if (time_sweep) HIPCHECK(hipEventRecord(t2, stream));

@@ -766,18 +766,74 @@ __global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict_
 // blocks per half sweep, and each thread evaluates one block for four sites at once.
 // ============================================================================================
 
-// SSC sweeps of one replica held in LDS (lat8: sublattice A then B).
-template <int NT, bool ALWAYS1>
+// Stream walk for lattices with at most K sites per stream and half sweep (N/2 <= 128 K):
+// the sites' offsets stay in registers for the whole step, packed two 16-bit offsets per
+// register, and the stream's words are consumed in order, word p = (2s + h) n_t + k, so a
+// running counter replaces the index arithmetic and a Philox block is evaluated every
+// fourth word.
+template <int K, bool ALWAYS1>
+__device__ __forceinline__ void ssc_walk_cached(int8_t* lat8, const RunDesc& d, uint64_t seed,
+                                                uint64_t sub0, uint32_t nsw, const uint32_t* tab,
+                                                int t, int nt) {
+  const int Nh = d.Nh;
+  uint32_t pa[2][K], pb[2][K];  // idx | i1 << 16, i3 | i4 << 16
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const Site q = site_at(k < nt ? t + kStreamsPerUnit * k : 0, h, d);
+      pa[h][k] = (uint32_t)q.idx | ((uint32_t)q.i1 << 16);
+      pb[h][k] = (uint32_t)q.i3 | ((uint32_t)q.i4 << 16);
+    }
+  U4 rb{0, 0, 0, 0};
+  uint64_t p = 0;
+  for (uint32_t s = 0; s < nsw; ++s) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      int8_t* own = lat8 + h * Nh;
+      const int8_t* oth = lat8 + (h ^ 1) * Nh;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (k < nt) {
+          if ((p & 3) == 0) rb = stream_block(seed, sub0 + t, p >> 2);
+          Site q;
+          q.idx = (int)(pa[h][k] & 0xFFFFu);
+          q.i1 = (int)(pa[h][k] >> 16);
+          q.i3 = (int)(pb[h][k] & 0xFFFFu);
+          q.i4 = (int)(pb[h][k] >> 16);
+          ssc_flip<ALWAYS1>(own, oth, q, u4_word(rb, (int)(p & 3)), tab);
+          ++p;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// SSC sweeps of one replica held in LDS (lat8: sublattice A then B).  PATH selects the code
+// compiled into the kernel (chosen per run on the host by ssc_path), so that the register-
+// caching paths do not raise the register count (and lower the occupancy) of the fast path:
+//   0: N/2 a multiple of 512 with at most NT/128 Philox blocks per thread and half sweep
+//      (fast), or the uncached block loop / stream walk for anything larger;
+//   1: aligned, two or three blocks per thread (register-cached block loop);
+//   2, 3: not aligned, at most 4 / 9 sites per stream (register-cached stream walk).
+template <int NT, bool ALWAYS1, int PATH>
 __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint64_t seed,
                                            uint64_t sub0, uint32_t nsw,
                                            const uint32_t* tab) {
   const int Nh = d.Nh;
+  if constexpr (PATH == 2 || PATH == 3) {
+    const int t = threadIdx.x;
+    const int nt = (t < kStreamsPerUnit && t < Nh) ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
+    ssc_walk_cached<PATH == 2 ? 4 : 9, ALWAYS1>(lat8, d, seed, sub0, nsw, tab, t, nt);
+    return;
+  }
   if ((Nh & 511) == 0) {
     // aligned: thread = (stream t, part); blocks m = part, part + P, ... of nb per half
     constexpr int P = NT / kStreamsPerUnit;
     const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
     const int nb = Nh / 512;
-    if (nb <= P) {
+    if (PATH == 0 && nb <= P) {
       // at most one block per half sweep per thread: its four sites stay in registers
       const bool act = part < nb;
       Site qs[2][4];
@@ -797,6 +853,48 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
             const int8_t* oth = lat8 + (h ^ 1) * Nh;
             const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + part);
             ssc_flip4<ALWAYS1>(own, oth, qs[h], sp[h], rb, tab);
+          }
+          __syncthreads();
+        }
+      }
+    } else if (PATH == 1) {
+      // up to three blocks per thread and half sweep (L = 96 at 512 threads): the twelve
+      // sites' offsets stay in registers, packed two 16-bit offsets per register
+      constexpr int MB = 3;
+      uint32_t pa[2][MB][4], pb[2][MB][4];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < MB; ++j)
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const int m = part + P * j;
+            const Site q = site_at(m < nb ? t + kStreamsPerUnit * (4 * m + jj) : 0, h, d);
+            pa[h][j][jj] = (uint32_t)q.idx | ((uint32_t)q.i1 << 16);
+            pb[h][j][jj] = (uint32_t)q.i3 | ((uint32_t)q.i4 << 16);
+          }
+      for (uint32_t s = 0; s < nsw; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          int8_t* own = lat8 + h * Nh;
+          const int8_t* oth = lat8 + (h ^ 1) * Nh;
+#pragma unroll
+          for (int j = 0; j < MB; ++j) {
+            const int m = part + P * j;
+            if (m < nb) {
+              const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+              Site q[4];
+              int sp[4];
+#pragma unroll
+              for (int jj = 0; jj < 4; ++jj) {
+                q[jj].idx = (int)(pa[h][j][jj] & 0xFFFFu);
+                q[jj].i1 = (int)(pa[h][j][jj] >> 16);
+                q[jj].i3 = (int)(pb[h][j][jj] & 0xFFFFu);
+                q[jj].i4 = (int)(pb[h][j][jj] >> 16);
+                sp[jj] = own[q[jj].idx];
+              }
+              ssc_flip4<ALWAYS1>(own, oth, q, sp, rb, tab);
+            }
           }
           __syncthreads();
         }
@@ -843,7 +941,7 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
 }
 
 // SSC: one block per replica.
-template <int NT>
+template <int NT, int PATH>
 __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ runs,
                                                   const int32_t* __restrict__ offs, int nruns) {
   const int* off = offs + kKUnits * (nruns + 1);
@@ -870,9 +968,9 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
   }
   __syncthreads();
   if (V1 > 0xFFFFFFFFull)
-    ssc_sweeps<NT, true>(lat8, d, d.seed, sub0, d.eqsweeps, tab);
+    ssc_sweeps<NT, true, PATH>(lat8, d, d.seed, sub0, d.eqsweeps, tab);
   else
-    ssc_sweeps<NT, false>(lat8, d, d.seed, sub0, d.eqsweeps, tab);
+    ssc_sweeps<NT, false, PATH>(lat8, d, d.seed, sub0, d.eqsweeps, tab);
   // energy and magnetisation (energyKer SSC:195-215)
   int e = 0, mm = 0;
   for (int idx = threadIdx.x; idx < Nh; idx += NT) {
@@ -1334,9 +1432,14 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW(k_init_ssc);
   PA_ALLOW(k_resample_msc<uint32_t>);
   PA_ALLOW(k_resample_msc<uint64_t>);
-  PA_ALLOW(k_sweep_ssc<128>);
-  PA_ALLOW(k_sweep_ssc<256>);
-  PA_ALLOW(k_sweep_ssc<512>);
+  PA_ALLOW((k_sweep_ssc<128, 0>));
+  PA_ALLOW((k_sweep_ssc<256, 0>));
+  PA_ALLOW((k_sweep_ssc<512, 0>));
+  PA_ALLOW((k_sweep_ssc<128, 1>));
+  PA_ALLOW((k_sweep_ssc<256, 1>));
+  PA_ALLOW((k_sweep_ssc<512, 1>));
+  PA_ALLOW((k_sweep_ssc<128, 2>));
+  PA_ALLOW((k_sweep_ssc<128, 3>));
 #define PA_ALLOW_MSC(WT, ST)                          \
   PA_ALLOW(k_sweep_msc_bal<WT, ST, 128>);             \
   PA_ALLOW(k_sweep_msc_bal<WT, ST, 256>);             \
@@ -1407,9 +1510,20 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 3: hipLaunchKernelGGL(k_resample, dim3(nblocks), dim3(256), 0, s, runs, offs, nruns); break;
     case 4:
       if (coding == kSSC) {
-        if (nt == 512) hipLaunchKernelGGL(k_sweep_ssc<512>, dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
-        else if (nt == 256) hipLaunchKernelGGL(k_sweep_ssc<256>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
-        else hipLaunchKernelGGL(k_sweep_ssc<128>, dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+        // for SSC the sw argument carries the sweep path (ssc_sweeps)
+        if (sw == 2 || sw == 3) {
+          if (nt != 128) return hipErrorInvalidValue;
+          if (sw == 2) hipLaunchKernelGGL((k_sweep_ssc<128, 2>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+          else hipLaunchKernelGGL((k_sweep_ssc<128, 3>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+        } else if (sw == 1) {
+          if (nt == 512) hipLaunchKernelGGL((k_sweep_ssc<512, 1>), dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
+          else if (nt == 256) hipLaunchKernelGGL((k_sweep_ssc<256, 1>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+          else hipLaunchKernelGGL((k_sweep_ssc<128, 1>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+        } else {
+          if (nt == 512) hipLaunchKernelGGL((k_sweep_ssc<512, 0>), dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
+          else if (nt == 256) hipLaunchKernelGGL((k_sweep_ssc<256, 0>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+          else hipLaunchKernelGGL((k_sweep_ssc<128, 0>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+        }
       } else {
         return launch_sweep_msc<false>(w, sw, nt, nblocks, lds, s, runs, offs, nruns);
       }
