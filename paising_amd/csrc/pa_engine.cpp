@@ -156,8 +156,29 @@ TuneKey tune_key(const pa_params& p) {
   return TuneKey{p.coding, p.coding == PA_MSC ? p.msbits : 1, p.L, rb};
 }
 
-// Validate, size and compute the per-step tables of a run (no device work).
-int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma) {
+// Default MSC slice width.  A sweep unit is one (group, slice) pair and runs on one CU; its
+// cost is about N (27 + 5 sw) VALU ops per sweep (27 per word, 5 per bit).  Narrow slices
+// give more units but repeat the per-word work, so the width minimises
+// max(one unit, all units / CUs): full words when the launch already fills the chip, narrow
+// slices for a few groups.  groups counts every group of the launch in units of this
+// lattice's sites.
+int device_cus();
+int default_slice(int w, int N, int64_t groups) {
+  const double cus = device_cus();
+  int best = 0;
+  double tbest = 0;
+  for (int sw = w; sw >= 8; sw /= 2) {
+    if ((size_t)N * (sw / 8) > 128 * 1024) continue;
+    const double c = 27.0 + 5.0 * sw;
+    const double t = std::max(c, (double)groups * (w / sw) * c / cus);
+    if (!best || t < tbest) { best = sw; tbest = t; }
+  }
+  return best ? best : 8;
+}
+
+// Validate, size and compute the per-step tables of a run (no device work).  co_work: sweep
+// work of the other runs that share this run's launches (groups x sites), 0 for a lone run.
+int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_work = 0) {
   pa_params p = p_in;
   {
     auto it = tune_cache().find(tune_key(p));
@@ -239,10 +260,7 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma) {
   }
   if (p.coding == PA_MSC) {
     int sw = p.slice_bits;
-    if (sw == 0) {
-      sw = h.w;
-      while (sw > 8 && (size_t)h.N * (sw / 8) > 128 * 1024) sw /= 2;
-    }
+    if (sw == 0) sw = default_slice(h.w, h.N, ceil_div(p.Rinit, h.w) + co_work / h.N);
     if (sw != 8 && sw != 16 && sw != 32 && sw != 64) return fail(PA_ERR_INVALID, "slice_bits");
     if (sw > h.w) sw = h.w;
     if ((size_t)h.N * (sw / 8) + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice slice exceeds LDS");
@@ -1225,10 +1243,18 @@ int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out
   auto b = std::make_unique<pa_batch>();
   int rc;
   if ((rc = b->batch.create_stream())) return rc;
+  // MSC runs of one word width share the sweep launches
+  auto msc_work = [&](int i) {
+    return p[i].coding == PA_MSC && p[i].msbits > 0
+               ? (int64_t)ceil_div(p[i].Rinit, p[i].msbits) * p[i].L * p[i].L : (int64_t)0;
+  };
   for (int i = 0; i < nruns; ++i) {
     if (p[i].adaptive) return fail(PA_ERR_INVALID, "adaptive runs take the plain path (pa_run_*)");
+    int64_t co = 0;
+    for (int j = 0; j < nruns; ++j)
+      if (j != i && p[j].msbits == p[i].msbits) co += msc_work(j);
     auto h = std::make_unique<RunHost>();
-    if ((rc = plan_run(p[i], *h, 10.0))) return rc;
+    if ((rc = plan_run(p[i], *h, 10.0, co))) return rc;
     if ((rc = materialize(*h))) return rc;
     b->batch.runs.push_back(h.get());
     b->batch.owned.push_back(std::move(h));
@@ -1345,7 +1371,8 @@ int pa_server_submit(pa_server* s, const pa_params* p, int64_t* ticket) {
   rq.ticket = s->next_ticket++;
   rq.p = *p;
   rq.h = std::make_unique<RunHost>();
-  int rc = plan_run(*p, *rq.h, 10.0);
+  // served runs share the chip with the other requests: widest slices (least total work)
+  int rc = plan_run(*p, *rq.h, 10.0, (int64_t)1 << 40);
   if (rc) return rc;
   rq.footprint = layout(*rq.h, nullptr);
   if (rq.footprint > s->budget) return fail(PA_ERR_MEMORY, "request exceeds the HBM budget");
