@@ -246,8 +246,10 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   // aligned lattices: 128 streams x P parts, P = blocks per stream per half sweep, <= 4
   const int nb = Nh / 512;
   h.nt = p.threads ? p.threads : (aligned ? 128 * std::min(4, std::max(1, nb)) : 128);
-  if (h.nt != 128 && h.nt != 256 && h.nt != 512)
-    return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512");
+  // 1024-thread workgroups (MSC only): twice the waves per CU for lattices whose slice
+  // leaves room for one workgroup per CU
+  if (h.nt != 128 && h.nt != 256 && h.nt != 512 && !(h.nt == 1024 && p.coding == PA_MSC))
+    return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512 (or 1024 for MSC)");
   if (!aligned) h.nt = 128;
   // SSC sweep kernel path (pa_kernels.hip, ssc_sweeps): register-cached block loop for two
   // or three Philox blocks per thread and half sweep, register-cached stream walk for
@@ -265,6 +267,9 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     if (sw > h.w) sw = h.w;
     if ((size_t)h.N * (sw / 8) + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice slice exceeds LDS");
     h.sw = sw;
+    // a slice over 80 KiB leaves room for one workgroup per CU: 1024 threads keep 4 waves
+    // per SIMD busy where 512 would leave 2
+    if (!p.threads && aligned && nb >= 8 && (size_t)h.N * (sw / 8) > 80 * 1024) h.nt = 1024;
     // resampling stages a 64-byte chunk of every parent group in LDS (<= 96 KiB); the
     // chunks of one 128-byte line are resampled on one XCD so the line is fetched once
     h.rs_sites = 64 / (h.w / 8);
@@ -406,7 +411,7 @@ int device_cus() {
 // count so that a share always exceeds one unit (a unit is cut at most once).
 void plan_balance(Group& g, const std::vector<RunHost*>& runs) {
   g.bal_workers = 0;
-  if (g.v.coding != PA_MSC) return;
+  if (g.v.coding != PA_MSC || g.v.nt > 512) return;
   int64_t units = 0;
   int force = 0;
   for (int m : g.members) {
@@ -1185,16 +1190,17 @@ int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
   std::vector<std::array<int, 3>> cands;
   const int Nh = p->L * p->L / 2;
   const bool aligned = (Nh % 512) == 0;
-  const int nts[3] = {128, 256, 512};
+  const int nts[4] = {128, 256, 512, 1024};
   for (int nt : nts) {
     if (!aligned && nt != 128) continue;
+    if (nt == 1024 && p->coding == PA_SSC) continue;
     if (p->coding == PA_SSC) {
       cands.push_back({0, nt, 0});
     } else {
       for (int sw = p->msbits; sw >= 8; sw /= 2) {
         if ((size_t)p->L * p->L * (sw / 8) + 2048 > kLdsLimit) continue;
         cands.push_back({sw, nt, -1});
-        cands.push_back({sw, nt, 1});
+        if (nt <= 512) cands.push_back({sw, nt, 1});
       }
     }
   }

@@ -1048,6 +1048,47 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
           __syncthreads();
         }
       }
+    } else if (d.r128 == 0) {
+      // Ld2 divides 128: every site of thread t lies in column t % Ld2, and site
+      // t + 128 k sits in row t / Ld2 + k q128.  Row parities within a block repeat from
+      // block to block, so the horizontal neighbour's offset is fixed per (half, jj); the
+      // vertical neighbours wrap only above jj = 0 (row 0) and below jj = 3 (row L - 1).
+      // All offsets are bytes into the LDS slice.
+      constexpr int B = (int)sizeof(S);
+      const int tx = t % Ld2, ty0 = t / Ld2;
+      const int dR = (tx + 1 == Ld2 ? 1 - Ld2 : 1) * B, dL = (tx == 0 ? Ld2 - 1 : -1) * B;
+      const int dV = Ld2 * B, dN = Nh * B;
+      int d1[2][4];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) d1[h][jj] = ((ty0 + jj * d.q128 + h) & 1) ? dR : dL;
+      for (uint32_t s = s_begin; s < s_end; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          uint8_t* own = (uint8_t*)(lat + h * Nh);
+          const uint8_t* oth = (const uint8_t*)(lat + (h ^ 1) * Nh);
+          for (int m = part; m < nb; m += P) {
+            const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+            uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
+            S c4[4], c8[4];
+            lcg_masks<S, 4>(ran, T4, T8, c4, c8);
+            const int o0 = (t + 512 * m) * B;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+              const int o = o0 + jj * 128 * B;
+              int o3 = o - dV, o4 = o + dV;
+              if (jj == 0 && o < dV) o3 += dN;
+              if (jj == 3 && o4 >= dN) o4 -= dN;
+              const S sp = *(const S*)(own + o);
+              *(S*)(own + o) = sp ^ msc_acc<S>(sp, *(const S*)(oth + o + d1[h][jj]),
+                                               *(const S*)(oth + o), *(const S*)(oth + o3),
+                                               *(const S*)(oth + o4), c4[jj], c8[jj]);
+            }
+          }
+          __syncthreads();
+        }
+      }
     } else {
       for (uint32_t s = s_begin; s < s_end; ++s) {
         for (int h = 0; h < 2; ++h) {
@@ -1348,7 +1389,8 @@ static hipError_t launch_sweep_msc(int w, int sw, int nt, int nblocks, size_t ld
                      offs, nruns)
 #define PA_MSC_NT(WT, ST)                   \
   do {                                      \
-    if (nt == 512) PA_MSC(WT, ST, 512);     \
+    if (nt == 1024) PA_MSC(WT, ST, 1024);   \
+    else if (nt == 512) PA_MSC(WT, ST, 512); \
     else if (nt == 256) PA_MSC(WT, ST, 256); \
     else PA_MSC(WT, ST, 128);               \
   } while (0)
@@ -1449,7 +1491,9 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW(k_sweep_msc<WT, ST, 512, false>);          \
   PA_ALLOW(k_sweep_msc<WT, ST, 128, true>);           \
   PA_ALLOW(k_sweep_msc<WT, ST, 256, true>);           \
-  PA_ALLOW(k_sweep_msc<WT, ST, 512, true>)
+  PA_ALLOW(k_sweep_msc<WT, ST, 512, true>);           \
+  PA_ALLOW(k_sweep_msc<WT, ST, 1024, false>);         \
+  PA_ALLOW(k_sweep_msc<WT, ST, 1024, true>)
   PA_ALLOW_MSC(uint32_t, uint32_t);
   PA_ALLOW_MSC(uint32_t, uint16_t);
   PA_ALLOW_MSC(uint32_t, uint8_t);

@@ -66,7 +66,9 @@ def test_balanced_batch_of_mixed_sizes():
 
 @pytest.mark.parametrize("L,w,sw,nt", [(32, 32, 16, 128), (32, 32, 8, 256), (64, 32, 16, 512),
                                        (32, 64, 32, 256), (32, 64, 16, 512), (48, 32, 8, 128),
-                                       (64, 32, 32, 128), (64, 32, 32, 256)])
+                                       (64, 32, 32, 128), (64, 32, 32, 256), (64, 32, 16, 1024),
+                                       (128, 64, 8, 1024), (32, 32, 32, 1024), (128, 32, 32, 256),
+                                       (256, 64, 16, 1024), (256, 32, 16, 512), (96, 32, 32, 256)])
 def test_slices_and_block_shapes_match_oracle(L, w, sw, nt):
     """Every replica-bit slice width and block shape the autotuner may pick is bit-exact."""
     args = (L, pa.MSC, w, 700, 4, 0.01, 1.0, 13, 4)
