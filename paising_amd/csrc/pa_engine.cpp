@@ -245,12 +245,15 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   const bool aligned = (Nh % 512) == 0;
   // aligned lattices: 128 streams x P parts, P = blocks per stream per half sweep, <= 4
   const int nb = Nh / 512;
-  h.nt = p.threads ? p.threads : (aligned ? 128 * std::min(4, std::max(1, nb)) : 128);
+  // unaligned MSC lattices: the Philox blocks of a stream's half sweep are dealt to P threads
+  const int nblk_u = ceil_div(ceil_div(Nh, 128), 4) + 1;
+  const int P_u = nblk_u >= 8 ? 4 : nblk_u >= 4 ? 2 : 1;
+  h.nt = p.threads ? p.threads : 128 * (aligned ? std::min(4, std::max(1, nb)) : P_u);
   // 1024-thread workgroups (MSC only): twice the waves per CU for lattices whose slice
   // leaves room for one workgroup per CU
   if (h.nt != 128 && h.nt != 256 && h.nt != 512 && !(h.nt == 1024 && p.coding == PA_MSC))
     return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512 (or 1024 for MSC)");
-  if (!aligned) h.nt = 128;
+  if (!aligned && p.coding == PA_SSC) h.nt = 128;
   // SSC sweep kernel path (pa_kernels.hip, ssc_sweeps): register-cached block loop for two
   // or three Philox blocks per thread and half sweep, register-cached stream walk for
   // unaligned lattices with at most 4 / 9 sites per stream and half sweep
@@ -269,7 +272,8 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     h.sw = sw;
     // a slice over 80 KiB leaves room for one workgroup per CU: 1024 threads keep 4 waves
     // per SIMD busy where 512 would leave 2
-    if (!p.threads && aligned && nb >= 8 && (size_t)h.N * (sw / 8) > 80 * 1024) h.nt = 1024;
+    if (!p.threads && (aligned ? nb >= 8 : nblk_u >= 16) && (size_t)h.N * (sw / 8) > 80 * 1024)
+      h.nt = 1024;
     // resampling stages a 64-byte chunk of every parent group in LDS (<= 96 KiB); the
     // chunks of one 128-byte line are resampled on one XCD so the line is fetched once
     h.rs_sites = 64 / (h.w / 8);
@@ -1192,8 +1196,7 @@ int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
   const bool aligned = (Nh % 512) == 0;
   const int nts[4] = {128, 256, 512, 1024};
   for (int nt : nts) {
-    if (!aligned && nt != 128) continue;
-    if (nt == 1024 && p->coding == PA_SSC) continue;
+    if (p->coding == PA_SSC && (nt == 1024 || (!aligned && nt != 128))) continue;
     if (p->coding == PA_SSC) {
       cands.push_back({0, nt, 0});
     } else {

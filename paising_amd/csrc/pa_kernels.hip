@@ -1105,24 +1105,37 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
       }
     }
   } else {
-    const int t = threadIdx.x;
-    const bool has = t < kStreamsPerUnit && t < Nh;
-    const int nt = has ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
-    U4 rb{0, 0, 0, 0};
-    uint64_t cached = ~0ull;
+    // Half sweeps that do not split into whole Philox blocks: stream t's n_t sites take the
+    // consecutive words (2s + h) n_t + k, k < n_t, of its stream.  The blocks spanning those
+    // words are dealt round robin to the P = NT / 128 threads of the stream; each block's
+    // four words drive four independent LCG chains, and a word outside the half sweep
+    // (first or last block) is computed and dropped.
+    constexpr int P = NT / kStreamsPerUnit;
+    const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
+    const int nts = t < Nh ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
     for (uint32_t s = s_begin; s < s_end; ++s) {
       for (int h = 0; h < 2; ++h) {
         S* own = lat + h * Nh;
         const S* oth = lat + (h ^ 1) * Nh;
-        for (int k = 0; k < nt; ++k) {
-          const uint64_t p = (uint64_t)(2 * s + h) * nt + k;
-          if ((p >> 2) != cached) { cached = p >> 2; rb = stream_block(seed, sub0 + t, cached); }
-          uint32_t ran[1] = {JA * u4_word(rb, (int)(p & 3)) + JC};
-          S c4[1], c8[1];
-          lcg_masks<S, 1>(ran, T4, T8, c4, c8);
-          const Site q = site_at(t + kStreamsPerUnit * k, h, d);
-          const S sp = own[q.idx];
-          own[q.idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[q.idx], oth[q.i3], oth[q.i4], c4[0], c8[0]);
+        const uint64_t base = (uint64_t)(2 * s + h) * nts;
+        const uint64_t b0 = base >> 2;
+        const int nblk = nts ? (int)(((base + nts - 1) >> 2) - b0 + 1) : 0;
+        for (int c = part; c < nblk; c += P) {
+          const U4 rb = stream_block(seed, sub0 + t, b0 + c);
+          uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
+          S c4[4], c8[4];
+          lcg_masks<S, 4>(ran, T4, T8, c4, c8);
+          const int k0 = (int)(((b0 + c) << 2) - base);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int k = k0 + j;
+            if (k >= 0 && k < nts) {
+              const Site q = site_at(t + kStreamsPerUnit * k, h, d);
+              const S sp = own[q.idx];
+              own[q.idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[q.idx], oth[q.i3], oth[q.i4],
+                                           c4[j], c8[j]);
+            }
+          }
         }
         __syncthreads();
       }
