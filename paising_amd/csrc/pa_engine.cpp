@@ -247,7 +247,7 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   const int nb = Nh / 512;
   // unaligned MSC lattices: the Philox blocks of a stream's half sweep are dealt to P threads
   const int nblk_u = ceil_div(ceil_div(Nh, 128), 4) + 1;
-  const int P_u = nblk_u >= 8 ? 4 : nblk_u >= 4 ? 2 : 1;
+  const int P_u = Nh <= 512 ? 1 : nblk_u >= 8 ? 4 : nblk_u >= 4 ? 2 : 1;
   h.nt = p.threads ? p.threads : 128 * (aligned ? std::min(4, std::max(1, nb)) : P_u);
   // 1024-thread workgroups (MSC only): twice the waves per CU for lattices whose slice
   // leaves room for one workgroup per CU

@@ -1104,6 +1104,30 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
         }
       }
     }
+  } else if (Nh <= 4 * kStreamsPerUnit) {
+    // at most 4 sites per stream and half sweep: one thread per stream walks its sites in
+    // word order with a Philox block cached across half sweeps
+    const int t = threadIdx.x;
+    const int nts = t < kStreamsPerUnit && t < Nh ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
+    U4 rb{0, 0, 0, 0};
+    uint64_t cached = ~0ull;
+    for (uint32_t s = s_begin; s < s_end; ++s) {
+      for (int h = 0; h < 2; ++h) {
+        S* own = lat + h * Nh;
+        const S* oth = lat + (h ^ 1) * Nh;
+        for (int k = 0; k < nts; ++k) {
+          const uint64_t p = (uint64_t)(2 * s + h) * nts + k;
+          if ((p >> 2) != cached) { cached = p >> 2; rb = stream_block(seed, sub0 + t, cached); }
+          uint32_t ran[1] = {JA * u4_word(rb, (int)(p & 3)) + JC};
+          S c4[1], c8[1];
+          lcg_masks<S, 1>(ran, T4, T8, c4, c8);
+          const Site q = site_at(t + kStreamsPerUnit * k, h, d);
+          const S sp = own[q.idx];
+          own[q.idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[q.idx], oth[q.i3], oth[q.i4], c4[0], c8[0]);
+        }
+        __syncthreads();
+      }
+    }
   } else {
     // Half sweeps that do not split into whole Philox blocks: stream t's n_t sites take the
     // consecutive words (2s + h) n_t + k, k < n_t, of its stream.  The blocks spanning those
