@@ -1207,8 +1207,10 @@ __device__ __forceinline__ int64_t run_units(const RunDesc& d, int NS) {
   return (int64_t)ceil_div(st->Rnew, d.w) * NS;
 }
 
+// At most 128 VGPRs (4 waves per SIMD), so two 512-thread workers fit a CU; the few
+// registers this spills are cheaper than running one worker per CU (7 % at L=64).
 template <typename W, typename S, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_msc_bal(const RunDesc* __restrict__ runs,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep_msc_bal(const RunDesc* __restrict__ runs,
                                                       int nruns, int P) {
   constexpr int SW = (int)sizeof(S) * 8;
   constexpr int NS = (int)sizeof(W) * 8 / SW;
