@@ -261,7 +261,9 @@ def serve_requests(n, seed=5):
 
 def bench_serve(args):
     reqs = serve_requests(args.nreq)
-    srv = pa.Server(hbm_budget_bytes=200 * 10**9, max_runs=256, steps_per_tick=args.tick_steps)
+    policy = {"shortest": pa.SERVE_SHORTEST, "share": pa.SERVE_SHARE}[args.policy]
+    srv = pa.Server(hbm_budget_bytes=200 * 10**9, max_runs=256, steps_per_tick=args.tick_steps,
+                    policy=policy)
     t0 = time.perf_counter()
     pending = list(reqs)
     tickets = []
@@ -287,9 +289,11 @@ def bench_serve(args):
         "metric": "served PA runs: p50/p99 completion latency (Poisson 20 req/s arrivals)",
         "value": round(pct(0.5), 4), "unit": "s (p50 latency)", "p99_s": round(pct(0.99), 4),
         "runs_per_s": round(len(lat) / wall, 4), "n_requests": len(lat), "n_gpus": 1,
+        "wall_s": round(wall, 3), "latencies_s": [round(x, 3) for x in lat],
         "higher_is_better": False, "bit_exact_vs_plain_checked": args.check if exact else "FAILED",
         "config": {"workload": "Poisson 20 req/s, L in {16..256 step 16}, R log-uniform [1e3,1e5], "
-                               "dB in {0.001,0.005}, Bfin U[0.44,1], EqSw=50, MSC32, 200 GB budget"},
+                               "dB in {0.001,0.005}, Bfin U[0.44,1], EqSw=50, MSC32, 200 GB budget",
+                   "policy": args.policy, "steps_per_tick": args.tick_steps},
     }
 
 
@@ -309,6 +313,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=0, help="truncate batched runs (0 = full)")
     ap.add_argument("--nreq", type=int, default=16)
     ap.add_argument("--tick-steps", type=int, default=32)
+    ap.add_argument("--policy", default="shortest", choices=["shortest", "share"],
+                    help="server tick policy (serve mode)")
     ap.add_argument("--check", type=int, default=4, help="served runs re-run on the plain path")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -38,6 +38,8 @@ enum pa_coding { PA_SSC = 0, PA_MSC = 1 };
  * PA_UNIFORM_CUDA101 = (v+1)/2^32 reproduces the CUDA >= 10.1 reference outputs,
  * PA_UNIFORM_CUDA100 = (v+1/2)/2^32 the CUDA <= 10.0 ones. */
 enum pa_uniform { PA_UNIFORM_CUDA101 = 0, PA_UNIFORM_CUDA100 = 1 };
+/* Server tick policies (pa_server_set_policy). */
+enum pa_serve_policy { PA_SERVE_SHARE = 0, PA_SERVE_SHORTEST = 1 };
 
 /* Run parameters: the reference's command-line options (PAisingSSC.cu:386-400) plus the
  * compile-time L and MSbits (PAisingSSC.cu:27, PAisingMSC.cu:34), made runtime. */
@@ -177,6 +179,11 @@ int pa_server_stats(pa_server* s, int* queued, int* resident, size_t* hbm_used);
  * quota inversely proportional to its work per step (EqSw * N * Rinit), at least 1 and at
  * most steps_per_tick (default 32); use_graph: ticks replay a captured hipGraph. */
 int pa_server_set_chunk(pa_server* s, int steps_per_tick, int use_graph);
+/* Which resident runs share a tick.  PA_SERVE_SHARE: all of them (processor sharing).
+ * PA_SERVE_SHORTEST (default): the runs with the least remaining work (steps left x work per
+ * step), in that order, until their workgroups reach four per CU; the rest pause for the tick.
+ * Results do not depend on the policy.  No counterpart in the reference (serving only). */
+int pa_server_set_policy(pa_server* s, int policy);
 /* Seconds from submission (and from admission) to completion of a finished request. */
 int pa_server_latency(pa_server* s, int64_t ticket, double* submit_to_done_s,
                       double* admit_to_done_s);

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpaising.so")
 BIN_DIR = os.path.join(HERE, "bin")
 
 SSC, MSC = 0, 1
+SERVE_SHARE, SERVE_SHORTEST = 0, 1  # server tick policies (pa_server_set_policy)
 UNIFORM_CUDA101, UNIFORM_CUDA100 = 0, 1
 OK, ERR_INVALID, ERR_HIP, ERR_NBMAX, ERR_CAPACITY, ERR_MEMORY, ERR_STATE = range(7)
 
@@ -89,6 +90,7 @@ SIGNATURES = {
     "pa_server_results": (_I, [_P, ctypes.c_int64, _P, _I, _P]),
     "pa_server_stats": (_I, [_P, _P, _P, _P]),
     "pa_server_set_chunk": (_I, [_P, _I, _I]),
+    "pa_server_set_policy": (_I, [_P, _I]),
     "pa_server_latency": (_I, [_P, ctypes.c_int64, _P, _P]),
     "pa_server_destroy": (_I, [_P]),
     "pa_run_footprint": (_SZ, [_P]),
@@ -349,11 +351,14 @@ class Batch:
 class Server:
     """Continuous-batching server: submit runs, tick, poll, fetch results."""
 
-    def __init__(self, hbm_budget_bytes, max_runs=256, steps_per_tick=32, use_graph=True):
+    def __init__(self, hbm_budget_bytes, max_runs=256, steps_per_tick=32, use_graph=True,
+                 policy=None):
         h = ctypes.c_void_p()
         _check(lib().pa_server_create(hbm_budget_bytes, max_runs, ctypes.byref(h)))
         self.h = h
         _check(lib().pa_server_set_chunk(self.h, steps_per_tick, int(use_graph)))
+        if policy is not None:
+            _check(lib().pa_server_set_policy(self.h, policy))
 
     def submit(self, params):
         t = ctypes.c_int64(0)

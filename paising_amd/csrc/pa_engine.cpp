@@ -1348,8 +1348,11 @@ struct pa_server {
   int64_t next_ticket = 1;
   int completed = 0;
   int chunk = 32;
+  int policy = PA_SERVE_SHORTEST;
   bool dirty = false;
   std::vector<int32_t> pause, quota;
+  hipEvent_t tick_ev[2] = {nullptr, nullptr};  // end of the last two ticks' device work
+  int64_t ticks = 0;
 };
 
 extern "C" {
@@ -1361,6 +1364,7 @@ int pa_server_create(size_t hbm_budget_bytes, int max_runs, pa_server** out) {
   s->max_runs = max_runs;
   int rc = s->batch.create_stream();
   if (rc) return rc;
+  for (hipEvent_t& e : s->tick_ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   s->batch.use_graph = true;
   *out = s.release();
   return PA_OK;
@@ -1370,6 +1374,13 @@ int pa_server_set_chunk(pa_server* s, int steps_per_tick, int use_graph) {
   if (!s || steps_per_tick < 1) return fail(PA_ERR_INVALID, "bad chunk");
   s->chunk = steps_per_tick;
   s->batch.use_graph = use_graph != 0;
+  return PA_OK;
+}
+
+int pa_server_set_policy(pa_server* s, int policy) {
+  if (!s || (policy != PA_SERVE_SHARE && policy != PA_SERVE_SHORTEST))
+    return fail(PA_ERR_INVALID, "bad policy");
+  s->policy = policy;
   return PA_OK;
 }
 
@@ -1456,6 +1467,10 @@ static int server_admit(pa_server* s) {
 int pa_server_tick(pa_server* s, int* completed) {
   if (!s) return fail(PA_ERR_INVALID, "null server");
   int rc;
+  // At most two ticks in flight: the device always has the next tick queued, and the host
+  // sees arrivals and schedules within a tick of the device instead of queueing whole
+  // schedules ahead of it.
+  if (s->ticks >= 2) HIPCHECK(hipEventSynchronize(s->tick_ev[s->ticks & 1]));
   if ((rc = server_retire(s))) return rc;
   if ((rc = server_admit(s))) return rc;
   if (s->dirty) {
@@ -1470,19 +1485,44 @@ int pa_server_tick(pa_server* s, int* completed) {
     // part of its credit (at most `chunk`); it pauses on the device after its quota while
     // the tick replays the step max_i k_i times.  Cheap runs therefore finish in few ticks
     // instead of advancing in lockstep with the expensive ones.
+    //
+    // PA_SERVE_SHORTEST shares the tick only among the runs with the least remaining work
+    // (steps left x work per step), taken in that order until their sweep units (workgroups)
+    // reach four per CU, so the device stays full while short requests overtake long
+    // ones; the others pause for the tick.
     const size_t n = s->resident.size();
     auto cost = [&](int64_t t) {
       const RunHost& h = *s->reqs[t].h;
       return (double)h.p.eqsweeps * h.N * h.R0 + 1.0;
     };
+    std::vector<char> on(n, 1);
+    if (s->policy == PA_SERVE_SHORTEST) {
+      std::vector<size_t> order(n);
+      for (size_t i = 0; i < n; ++i) order[i] = i;
+      auto rem = [&](size_t i) {
+        const int64_t t = s->resident[i];
+        return cost(t) * s->reqs[t].steps_left;
+      };
+      std::stable_sort(order.begin(), order.end(),
+                       [&](size_t a, size_t b) { return rem(a) < rem(b); });
+      const int64_t fill = 4 * (int64_t)device_cus();
+      int64_t units = 0;
+      std::fill(on.begin(), on.end(), 0);
+      for (size_t j = 0; j < n && units < fill; ++j) {
+        const RunHost& h = *s->reqs[s->resident[order[j]]].h;
+        on[order[j]] = 1;
+        units += (int64_t)h.cap_units * (h.p.coding == PA_SSC ? 1 : h.w / h.sw);
+      }
+    }
     double cmax = 0;
-    for (int64_t t : s->resident) cmax = std::max(cmax, cost(t));
+    for (size_t i = 0; i < n; ++i)
+      if (on[i]) cmax = std::max(cmax, cost(s->resident[i]));
     s->pause.resize(n);
     s->quota.resize(n);
     int K = 0;
     for (size_t i = 0; i < n; ++i) {
       Request& rq = s->reqs[s->resident[i]];
-      rq.credit = std::min(rq.credit + cmax / cost(rq.ticket), (double)s->chunk);
+      if (on[i]) rq.credit = std::min(rq.credit + cmax / cost(rq.ticket), (double)s->chunk);
       const int k = std::min((int)rq.credit, rq.steps_left);
       rq.credit -= k;
       s->quota[i] = k;
@@ -1493,6 +1533,8 @@ int pa_server_tick(pa_server* s, int* completed) {
                             hipMemcpyHostToDevice, s->batch.stream));
     HIPCHECK(pa_launch_resume(s->batch.stream, s->batch.desc_d, (int)n, s->batch.pause_d));
     if ((rc = s->batch.run_steps(K))) return rc;
+    HIPCHECK(hipEventRecord(s->tick_ev[s->ticks & 1], s->batch.stream));
+    ++s->ticks;
     for (size_t i = 0; i < n; ++i) s->reqs[s->resident[i]].steps_left -= s->quota[i];
     if ((rc = server_retire(s))) return rc;
   }
@@ -1539,6 +1581,8 @@ int pa_server_stats(pa_server* s, int* queued, int* resident, size_t* hbm_used) 
 int pa_server_destroy(pa_server* s) {
   if (!s) return PA_OK;
   (void)hipStreamSynchronize(s->batch.stream);
+  for (hipEvent_t e : s->tick_ev)
+    if (e) (void)hipEventDestroy(e);
   s->batch.runs.clear();
   delete s;
   return PA_OK;

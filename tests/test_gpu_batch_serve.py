@@ -54,9 +54,10 @@ def test_batch_matches_plain(use_graph):
     b.close()
 
 
-def test_server_matches_plain_alone_and_in_mix():
+@pytest.mark.parametrize("policy", [pa.SERVE_SHORTEST, pa.SERVE_SHARE])
+def test_server_matches_plain_alone_and_in_mix(policy):
     plist = mix(10, 11)
-    srv = pa.Server(hbm_budget_bytes=64 * 2**20, max_runs=4, steps_per_tick=3)
+    srv = pa.Server(hbm_budget_bytes=64 * 2**20, max_runs=4, steps_per_tick=3, policy=policy)
     tickets = []
     for i, p in enumerate(plist):
         tickets.append(srv.submit(p))
@@ -73,13 +74,15 @@ def test_server_matches_plain_alone_and_in_mix():
     srv.close()
 
 
-def test_server_shares_device_time():
-    """A cheap request submitted after an expensive one finishes first (each resident run
-    gets about the same device time per tick), and both stay bit-exact."""
+@pytest.mark.parametrize("policy", [pa.SERVE_SHORTEST, pa.SERVE_SHARE])
+def test_server_shares_device_time(policy):
+    """A cheap request submitted after an expensive one finishes first (processor sharing:
+    each resident run gets about the same device time per tick; shortest remaining work: the
+    cheap run is scheduled first), and both stay bit-exact."""
     big = pa.make_params(L=64, coding=pa.MSC, msbits=32, Rinit=20000, eqsweeps=50, dB=0.005,
                          Bfin=1.0, seed=71)
     small = pa.make_params(L=16, coding=pa.SSC, Rinit=500, eqsweeps=5, dB=0.005, Bfin=1.0, seed=72)
-    srv = pa.Server(hbm_budget_bytes=1 << 30, max_runs=8, steps_per_tick=32)
+    srv = pa.Server(hbm_budget_bytes=1 << 30, max_runs=8, steps_per_tick=32, policy=policy)
     tb = srv.submit(big)
     ts = srv.submit(small)
     while not srv.done(ts):
@@ -89,6 +92,30 @@ def test_server_shares_device_time():
         srv.tick()
     same(pa.run_all(big)[0], srv.results(tb))
     same(pa.run_all(small)[0], srv.results(ts))
+    srv.close()
+
+
+def test_server_shortest_first_pauses_long_runs():
+    """Under the shortest-remaining-work policy, once short runs fill the device a long run
+    waits; it still finishes bit-exact."""
+    long_run = pa.make_params(L=64, coding=pa.MSC, msbits=32, Rinit=40000, eqsweeps=20,
+                              dB=0.05, Bfin=1.0, seed=81)
+    shorts = [pa.make_params(L=32, coding=pa.SSC, Rinit=2000, eqsweeps=5, dB=0.05, Bfin=1.0,
+                             seed=82 + i) for i in range(3)]
+    srv = pa.Server(hbm_budget_bytes=1 << 30, max_runs=8, steps_per_tick=4,
+                    policy=pa.SERVE_SHORTEST)
+    tl = srv.submit(long_run)
+    ts = [srv.submit(p) for p in shorts]
+    while not all(srv.done(t) for t in ts):
+        srv.tick()
+    assert not srv.done(tl)
+    while not srv.done(tl):
+        srv.tick()
+    same(pa.run_all(long_run)[0], srv.results(tl))
+    for t, p in zip(ts, shorts):
+        same(pa.run_all(p)[0], srv.results(t))
+    with pytest.raises(pa.PAError):
+        pa.Server(hbm_budget_bytes=1 << 20, policy=7)
     srv.close()
 
 
