@@ -168,7 +168,7 @@ int default_slice(int w, int N, int64_t groups) {
   int best = 0;
   double tbest = 0;
   for (int sw = w; sw >= 8; sw /= 2) {
-    if ((size_t)N * (sw / 8) > 128 * 1024) continue;
+    if ((size_t)N * (sw / 8) + 2048 > kLdsLimit) continue;  // the slice must fit LDS
     const double c = 27.0 + 5.0 * sw;
     const double t = std::max(c, (double)groups * (w / sw) * c / cus);
     if (!best || t < tbest) { best = sw; tbest = t; }
@@ -281,9 +281,10 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     h.rs_lp = 128 / (h.rs_sites * (h.w / 8));
     if ((size_t)h.cap_units * h.rs_sites * (h.w / 8) > 150 * 1024)
       return fail(PA_ERR_INVALID, "population too large for one device resampling chunk");
-    // about one new group per thread; the splits of one site chunk run on one XCD and
-    // re-read the chunk from that XCD's L2
-    h.rs_split = std::max(1, std::min(16, ceil_div(h.cap_units, 256)));
+    // about one new group per thread (1024 threads per workgroup for 32-bit words, 256 for
+    // 64-bit words); the splits of one site chunk run on one XCD and re-read the chunk from
+    // that XCD's L2
+    h.rs_split = std::max(1, std::min(16, ceil_div(h.cap_units, h.w == 32 ? 1024 : 256)));
   } else {
     if ((size_t)h.N + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice exceeds LDS");
     h.sw = 8;

@@ -691,8 +691,14 @@ __global__ __launch_bounds__(256) void k_resample(const RunDesc* __restrict__ ru
 // parents' chunks fit in LDS): the chunk of every parent group is staged in LDS once, then
 // every new word of the chunk gathers its w bits from there.  Each byte of the parent
 // population is read once and each byte of the children written once.
+// 1024 threads per workgroup for 32-bit words (one new group per thread: the large parent
+// chunk of big populations leaves room for one workgroup per CU, which 256 threads cannot
+// keep busy), 256 for 64-bit words (the 64 parent indices per thread need the registers).
 template <typename W>
-__global__ __launch_bounds__(256) void k_resample_msc(const RunDesc* __restrict__ runs,
+constexpr int resample_threads() { return sizeof(W) == 4 ? 1024 : 256; }
+
+template <typename W>
+__global__ __launch_bounds__(resample_threads<W>()) void k_resample_msc(const RunDesc* __restrict__ runs,
                                                       const int32_t* __restrict__ offs, int nruns) {
   constexpr int w = (int)sizeof(W) * 8;
   const int* off = offs + kKResample * (nruns + 1);
@@ -1621,8 +1627,8 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 13: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 0); break;
     case 14: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 1); break;
     case 10:
-      if (w == 32) hipLaunchKernelGGL(k_resample_msc<uint32_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
-      else hipLaunchKernelGGL(k_resample_msc<uint64_t>, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+      if (w == 32) hipLaunchKernelGGL(k_resample_msc<uint32_t>, dim3(nblocks), dim3(resample_threads<uint32_t>()), lds, s, runs, offs, nruns);
+      else hipLaunchKernelGGL(k_resample_msc<uint64_t>, dim3(nblocks), dim3(resample_threads<uint64_t>()), lds, s, runs, offs, nruns);
       break;
     default: return hipErrorInvalidValue;
   }

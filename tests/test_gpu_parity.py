@@ -71,7 +71,8 @@ def test_balanced_batch_of_mixed_sizes():
                                        (256, 64, 16, 1024), (256, 32, 16, 512), (96, 32, 32, 256),
                                        (48, 32, 32, 256), (80, 32, 32, 512), (144, 32, 16, 1024),
                                        (10, 32, 32, 256), (16, 64, 64, 512), (112, 32, 0, 0),
-                                       (176, 64, 0, 0), (24, 32, 0, 0), (40, 32, 0, 0)])
+                                       (176, 64, 0, 0), (24, 32, 0, 0), (40, 32, 0, 0),
+                                       (192, 32, 32, 1024), (192, 32, 0, 0), (160, 32, 0, 0)])
 def test_slices_and_block_shapes_match_oracle(L, w, sw, nt):
     """Every replica-bit slice width and block shape the autotuner may pick is bit-exact,
     on lattices whose half sweeps split into whole Philox blocks per stream and on lattices
