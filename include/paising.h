@@ -157,6 +157,36 @@ int pa_run_histogram(pa_run* r, int32_t* hist, int nbins);
 int pa_mhr_analysis(int N, int nB, const double* B, double* BF, const int32_t* R,
                     const int32_t* hist, int niter, double* out);
 
+/* ---- single stages of a beta step on caller data (host arrays in and out) ----
+ * Each call builds a scratch device run (it allocates), fills its state from the
+ * arguments, launches the kernels of that part of a step -- the same launches a run makes --
+ * and copies the results back.  p supplies L, coding, msbits, seed, the uniform mapping,
+ * EqSw, Rinit (copy counts) and the sweep variant; the other schedule fields are ignored.
+ * Layouts as pa_run_population: SSC R x N int8, MSC ceil(R/w) x N words; per-member arrays
+ * have R (SSC) or ceil(R/w)*w (MSC) entries; `active` holds one byte per MSC lane (NULL:
+ * lanes below R). */
+/* checkKerALL + energyKer (PAisingSSC.cu:159-215, PAisingMSC.cu:190-317): EqSw sweeps of R
+ * replicas at inverse temperature beta, sweep streams starting at subsequence `is`
+ * (unit B uses is + 128 B + t), then energies and magnetisations. */
+int pa_stage_sweep(const pa_params* p, int R, uint64_t is, double beta, void* pop,
+                   int32_t* IE, int32_t* M);
+/* energyKer alone (PAisingSSC.cu:195-215, PAisingMSC.cu:268-317). */
+int pa_stage_energy(const pa_params* p, int R, const void* pop, int32_t* IE, int32_t* M);
+/* QKer, lnQ, CalcTauKer and CalcParSum (PAisingSSC.cu:217-270,533; PAisingMSC.cu:319-395,
+ * 696): q and lnQ of parents with energies IE at increment dB around Emean, copy counts
+ * drawn from subsequences is + n, their exclusive prefix and R_new. */
+int pa_stage_copy_counts(const pa_params* p, int R, const int32_t* IE, const uint8_t* active,
+                         double dB, double Emean, uint64_t is, double* q, double* lnQ,
+                         uint32_t* Roff, uint32_t* prefix, int* Rnew);
+/* resampleKer (PAisingSSC.cu:272-280, PAisingMSC.cu:397-435): the children of R parents
+ * with copy counts Roff (MSC: child n -> group n mod rg_new, lane n div rg_new). */
+int pa_stage_resample(const pa_params* p, int R, const void* pop, const uint8_t* active,
+                      const uint32_t* Roff, void* pop_new, uint8_t* active_new, int* Rnew);
+/* CalcAverages (PAisingSSC.cu:282-292, PAisingMSC.cu:437-450): sums of E, E^2, |M|, M^2, M^4
+ * over the active members in the device's fixed reduction order, and the integer sums. */
+int pa_stage_averages(const pa_params* p, int R, const int32_t* IE, const int32_t* M,
+                      const uint8_t* active, double sums[5], int64_t* sumE, int64_t* sumM);
+
 /* ---- batched runs (no reference counterpart: many independent runs per launch) ---- */
 int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out);
 int pa_batch_run(pa_batch* b);            /* all runs to completion */

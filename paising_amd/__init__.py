@@ -91,6 +91,12 @@ SIGNATURES = {
     "pa_server_stats": (_I, [_P, _P, _P, _P]),
     "pa_server_set_chunk": (_I, [_P, _I, _I]),
     "pa_server_set_policy": (_I, [_P, _I]),
+    "pa_stage_sweep": (_I, [_P, _I, ctypes.c_uint64, ctypes.c_double, _P, _P, _P]),
+    "pa_stage_energy": (_I, [_P, _I, _P, _P, _P]),
+    "pa_stage_copy_counts": (_I, [_P, _I, _P, _P, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_uint64, _P, _P, _P, _P, _P]),
+    "pa_stage_resample": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "pa_stage_averages": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "pa_server_latency": (_I, [_P, ctypes.c_int64, _P, _P]),
     "pa_server_destroy": (_I, [_P]),
     "pa_run_footprint": (_SZ, [_P]),
@@ -157,6 +163,88 @@ def schedule_steps(Binit, dB, Bfin):
 
 def footprint(params):
     return lib().pa_run_footprint(ctypes.byref(params))
+
+
+def _members(params, R):
+    return R if params.coding == SSC else -(-R // params.msbits) * params.msbits
+
+
+def _pop_shape(params, R):
+    N = params.L * params.L
+    if params.coding == SSC:
+        return (R, N), np.int8
+    return (-(-R // params.msbits), N), (np.uint32 if params.msbits == 32 else np.uint64)
+
+
+def stage_sweep(params, pop, R, IS, beta):
+    """checkKerALL + energyKer on a copy of `pop`: returns (new pop, IE, M)."""
+    shape, dt = _pop_shape(params, R)
+    out = np.ascontiguousarray(np.array(pop, dtype=dt).reshape(shape))
+    IE = np.zeros(_members(params, R), np.int32)
+    M = np.zeros_like(IE)
+    _check(lib().pa_stage_sweep(ctypes.byref(params), R, IS, beta, _ptr(out), _ptr(IE), _ptr(M)))
+    return out, IE, M
+
+
+def stage_energy(params, pop, R):
+    shape, dt = _pop_shape(params, R)
+    src = np.ascontiguousarray(np.array(pop, dtype=dt).reshape(shape))
+    IE = np.zeros(_members(params, R), np.int32)
+    M = np.zeros_like(IE)
+    _check(lib().pa_stage_energy(ctypes.byref(params), R, _ptr(src), _ptr(IE), _ptr(M)))
+    return IE, M
+
+
+def stage_copy_counts(params, IE, R, dB, Emean, IS, active=None):
+    """QKer, lnQ, CalcTauKer, CalcParSum: returns (q, lnQ, Roff, prefix, Rnew)."""
+    n = _members(params, R)
+    ie = np.ascontiguousarray(np.array(IE, dtype=np.int32)[:n])
+    act = None if active is None else np.ascontiguousarray(np.array(active, dtype=np.uint8)[:n])
+    q, lnq = ctypes.c_double(0), ctypes.c_double(0)
+    Roff = np.zeros(n, np.uint32)
+    pre = np.zeros(n, np.uint32)
+    rn = ctypes.c_int(0)
+    _check(lib().pa_stage_copy_counts(ctypes.byref(params), R, _ptr(ie),
+                                      None if act is None else _ptr(act), dB, Emean, IS,
+                                      ctypes.byref(q), ctypes.byref(lnq), _ptr(Roff), _ptr(pre),
+                                      ctypes.byref(rn)))
+    return q.value, lnq.value, Roff, pre, rn.value
+
+
+def stage_resample(params, pop, R, Roff, active=None):
+    """resampleKer: returns (children population, MSC active bytes or None, Rnew)."""
+    shape, dt = _pop_shape(params, R)
+    src = np.ascontiguousarray(np.array(pop, dtype=dt).reshape(shape))
+    n = _members(params, R)
+    ro = np.ascontiguousarray(np.array(Roff, dtype=np.uint32)[:n])
+    act = None if active is None else np.ascontiguousarray(np.array(active, dtype=np.uint8)[:n])
+    Rn_max = int(ro.astype(np.int64).sum())
+    shape_n, _ = _pop_shape(params, max(Rn_max, 1))
+    out = np.zeros(shape_n, dt)
+    an = np.zeros(_members(params, max(Rn_max, 1)), np.uint8) if params.coding == MSC else None
+    rn = ctypes.c_int(0)
+    _check(lib().pa_stage_resample(ctypes.byref(params), R, _ptr(src),
+                                   None if act is None else _ptr(act), _ptr(ro), _ptr(out),
+                                   None if an is None else _ptr(an), ctypes.byref(rn)))
+    shape_r, _ = _pop_shape(params, rn.value)
+    out = out[:shape_r[0]]
+    if an is not None:
+        an = an[:_members(params, rn.value)]
+    return out, an, rn.value
+
+
+def stage_averages(params, IE, M, R, active=None):
+    """CalcAverages: returns (5 FP64 sums, sum E, sum M)."""
+    n = _members(params, R)
+    ie = np.ascontiguousarray(np.array(IE, dtype=np.int32)[:n])
+    m = np.ascontiguousarray(np.array(M, dtype=np.int32)[:n])
+    act = None if active is None else np.ascontiguousarray(np.array(active, dtype=np.uint8)[:n])
+    sums = (ctypes.c_double * 5)()
+    se, sm = ctypes.c_int64(0), ctypes.c_int64(0)
+    _check(lib().pa_stage_averages(ctypes.byref(params), R, _ptr(ie), _ptr(m),
+                                   None if act is None else _ptr(act), sums, ctypes.byref(se),
+                                   ctypes.byref(sm)))
+    return list(sums), se.value, sm.value
 
 
 def device_info():

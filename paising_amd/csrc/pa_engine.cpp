@@ -625,27 +625,35 @@ struct Batch {
     return PA_OK;
   }
 
-  int launch_step(bool time_sweep) {
+  // The parts of a step in launch order; the stage entry points launch them one at a time.
+  // q partials, lnQ and copy counts (QKer, CalcTauKer).
+  int launch_counts() {
     const int n = (int)runs.size();
-    hipEvent_t t0 = nullptr, t1 = nullptr, t2 = nullptr;
-    if (time_sweep) {
-      int rc;
-      if ((rc = timing_event(&t0)) || (rc = timing_event(&t1)) || (rc = timing_event(&t2))) return rc;
-    }
     if (has_w[0] || has_w[1])
       HIPCHECK(pa_launch_step(8, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(1, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
+    return PA_OK;
+  }
+  // exclusive prefix of the copy counts and the parent of every child (CalcParSum)
+  int launch_scan() {
+    const int n = (int)runs.size();
     HIPCHECK(pa_launch_step(2, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(9, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
-    if (time_sweep) HIPCHECK(hipEventRecord(t0, stream));
+    return PA_OK;
+  }
+  int launch_resample() {
+    const int n = (int)runs.size();
     if (has_w[0] || has_w[1] ? has_ssc() : true)
       HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, blocks[kKResample], 0, stream, desc_d, offs_d, n));
     for (int wi = 0; wi < 2; ++wi)
       if (has_w[wi])
         HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, 0, blocks[kKResample], rs_lds[wi], stream,
                                 desc_d, offs_d, n));
-    if (time_sweep) HIPCHECK(hipEventRecord(t1, stream));
+    return PA_OK;
+  }
+  // EqSw sweeps and energies, one launch per kernel variant
+  int launch_sweeps() {
     for (const Group& g : groups) {
       const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;
       if (g.bal_workers > 0)
@@ -656,12 +664,30 @@ struct Batch {
                                 g.units_blocks * ns, g.lds, stream,
                                 g.desc_d, g.offs_d, (int)g.members.size()));
     }
-    if (time_sweep) HIPCHECK(hipEventRecord(t2, stream));
+    return PA_OK;
+  }
+  // averages (CalcAverages), optional hashes and histograms, then the step record (k_fin)
+  int launch_finish() {
+    const int n = (int)runs.size();
     HIPCHECK(pa_launch_step(5, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n));
     if (any_trace) HIPCHECK(pa_launch_step(6, 0, 0, 0, 0, blocks[kKUnits], 0, stream, desc_d, offs_d, n));
     if (any_mhr) HIPCHECK(pa_launch_step(13, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(7, 0, 0, 0, 0, n, 0, stream, desc_d, offs_d, n));
     return PA_OK;
+  }
+
+  int launch_step(bool time_sweep) {
+    hipEvent_t t0 = nullptr, t1 = nullptr, t2 = nullptr;
+    int rc;
+    if (time_sweep)
+      if ((rc = timing_event(&t0)) || (rc = timing_event(&t1)) || (rc = timing_event(&t2))) return rc;
+    if ((rc = launch_counts()) || (rc = launch_scan())) return rc;
+    if (time_sweep) HIPCHECK(hipEventRecord(t0, stream));
+    if ((rc = launch_resample())) return rc;
+    if (time_sweep) HIPCHECK(hipEventRecord(t1, stream));
+    if ((rc = launch_sweeps())) return rc;
+    if (time_sweep) HIPCHECK(hipEventRecord(t2, stream));
+    return launch_finish();
   }
 
   int ensure_graph() {
@@ -1246,6 +1272,229 @@ int pa_autotune_clear(void) {
   tune_cache().clear();
   return PA_OK;
 }
+
+// ---------------- single stages on caller data ----------------
+// Each stage entry point builds a one-step scratch run, fills its state and arrays from the
+// caller's host arrays, launches the kernels of that part of a step (the same launches as a
+// step of a run) and copies the results back.
+namespace {
+
+struct StageRun {
+  RunHost h;
+  Batch b;  // declared after h: destroyed first
+};
+
+// units (replicas or groups) and members (replicas or group lanes) of a population of R
+int stage_units(const RunHost& h, int R) { return h.p.coding == PA_SSC ? R : ceil_div(R, h.w); }
+size_t stage_members(const RunHost& h, int R) {
+  return h.p.coding == PA_SSC ? (size_t)R : (size_t)ceil_div(R, h.w) * h.w;
+}
+size_t stage_unit_bytes(const RunHost& h) {
+  return h.p.coding == PA_SSC ? (size_t)h.N : (size_t)h.N * (h.w / 8);
+}
+
+// Scratch run with room for max(R, cap) members, step 1 at inverse temperature beta with
+// increment dB; the caller's Rinit is kept for the copy-count ratio.
+int stage_open(const pa_params* p, int R, int cap, double beta, double dB, StageRun& s) {
+  if (!p || R < 1 || cap < 1) return fail(PA_ERR_INVALID, "bad stage arguments");
+  pa_params q = *p;
+  q.Rinit = std::max(R, cap);
+  q.Binit = 0.0;
+  q.dB = 1.0;
+  q.Bfin = 1.0;
+  q.max_steps = 1;
+  q.adaptive = 0;
+  q.schedule = nullptr;
+  q.schedule_len = 0;
+  q.mhr = 0;
+  q.trace = 0;
+  int rc = plan_run(q, s.h, 10.0);
+  if (rc) return rc;
+  if ((rc = materialize(s.h))) return rc;
+  s.h.desc.Rinit = p->Rinit;
+  const double bt[2] = {0.0, beta}, db[2] = {0.0, dB};
+  uint64_t thr[3];
+  step_thresholds(s.h, beta, thr);
+  HIPCHECK(hipMemcpy((void*)s.h.desc.beta, bt, sizeof bt, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy((void*)s.h.desc.dbeta, db, sizeof db, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy((void*)(s.h.desc.thr + 3), thr, sizeof thr, hipMemcpyHostToDevice));
+  if ((rc = s.b.create_stream())) return rc;
+  s.b.runs = {&s.h};
+  return s.b.build_tables();
+}
+
+int stage_state(StageRun& s, int R, int Rnew, int buf, uint64_t IS, double Emean) {
+  RunState st{};
+  st.step = 1;
+  st.R = R;
+  st.Rnew = Rnew;
+  st.buf = buf;
+  st.IS = IS;
+  st.Emean = Emean;
+  st.active = 1;
+  HIPCHECK(hipMemcpy(s.h.desc.st, &st, sizeof st, hipMemcpyHostToDevice));
+  return PA_OK;
+}
+
+int stage_sync(StageRun& s, RunState* st) {
+  HIPCHECK(hipStreamSynchronize(s.b.stream));
+  HIPCHECK(hipMemcpy(st, s.h.desc.st, sizeof *st, hipMemcpyDeviceToHost));
+  if (st->err) return fail(PA_ERR_HIP, "device error bits " + std::to_string(st->err));
+  if (st->overflow) return fail(PA_ERR_CAPACITY, "children exceed the stage's capacity");
+  return PA_OK;
+}
+
+// one byte per lane <-> 64-bit lane mask per group
+int stage_put_masks(const RunHost& h, int which, const uint8_t* active, int R) {
+  const int rg = ceil_div(R, h.w);
+  std::vector<uint64_t> m(rg, 0);
+  for (int g = 0; g < rg; ++g)
+    for (int k = 0; k < h.w; ++k)
+      if (active ? active[(size_t)g * h.w + k] != 0 : (int64_t)g * h.w + k < R) m[g] |= 1ull << k;
+  HIPCHECK(hipMemcpy(h.desc.amask[which], m.data(), m.size() * 8, hipMemcpyHostToDevice));
+  return PA_OK;
+}
+int stage_get_masks(const RunHost& h, int which, uint8_t* active, int R) {
+  const int rg = ceil_div(R, h.w);
+  std::vector<uint64_t> m(rg, 0);
+  HIPCHECK(hipMemcpy(m.data(), h.desc.amask[which], m.size() * 8, hipMemcpyDeviceToHost));
+  for (int g = 0; g < rg; ++g)
+    for (int k = 0; k < h.w; ++k) active[(size_t)g * h.w + k] = (uint8_t)((m[g] >> k) & 1);
+  return PA_OK;
+}
+
+int stage_sweep(const pa_params* p, int R, uint64_t is, double beta, void* pop, int32_t* IE,
+                int32_t* M) {
+  if (!pop) return fail(PA_ERR_INVALID, "null population");
+  StageRun s;
+  int rc = stage_open(p, R, R, beta, 0.0, s);
+  if (rc) return rc;
+  const RunHost& h = s.h;
+  const size_t bytes = stage_unit_bytes(h) * stage_units(h, R);
+  HIPCHECK(hipMemcpy(h.desc.pop[0], pop, bytes, hipMemcpyHostToDevice));
+  // the sweep reads the population the resampling wrote (buffer buf ^ 1) and draws from
+  // subsequence IS + (parents' tau draws) + t + 128 unit; with no parents that is `is`
+  if ((rc = stage_state(s, 0, R, 1, is, 0.0))) return rc;
+  if ((rc = s.b.launch_sweeps())) return rc;
+  RunState st;
+  if ((rc = stage_sync(s, &st))) return rc;
+  HIPCHECK(hipMemcpy(pop, h.desc.pop[0], bytes, hipMemcpyDeviceToHost));
+  const size_t mem = stage_members(h, R);
+  if (IE) HIPCHECK(hipMemcpy(IE, h.desc.IE, mem * 4, hipMemcpyDeviceToHost));
+  if (M) HIPCHECK(hipMemcpy(M, h.desc.M, mem * 4, hipMemcpyDeviceToHost));
+  return PA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pa_stage_sweep(const pa_params* p, int R, uint64_t is, double beta, void* pop, int32_t* IE,
+                   int32_t* M) {
+  return stage_sweep(p, R, is, beta, pop, IE, M);
+}
+
+int pa_stage_energy(const pa_params* p, int R, const void* pop, int32_t* IE, int32_t* M) {
+  if (!p || !pop) return fail(PA_ERR_INVALID, "null argument");
+  pa_params q = *p;
+  q.eqsweeps = 0;
+  // zero sweeps leave the population unchanged; work on a copy to keep `pop` const
+  const size_t unit = p->coding == PA_SSC ? (size_t)p->L * p->L : (size_t)p->L * p->L * (p->msbits / 8);
+  const size_t units = p->coding == PA_SSC ? (size_t)R : (size_t)ceil_div(R, std::max(1, p->msbits));
+  std::vector<uint8_t> tmp((const uint8_t*)pop, (const uint8_t*)pop + unit * units);
+  return stage_sweep(&q, R, 0, 0.0, tmp.data(), IE, M);
+}
+
+int pa_stage_copy_counts(const pa_params* p, int R, const int32_t* IE, const uint8_t* active,
+                         double dB, double Emean, uint64_t is, double* q, double* lnQ,
+                         uint32_t* Roff, uint32_t* prefix, int* Rnew) {
+  if (!p || !IE) return fail(PA_ERR_INVALID, "null argument");
+  StageRun s;
+  int rc = stage_open(p, R, p->Rinit, 0.0, dB, s);
+  if (rc) return rc;
+  const RunHost& h = s.h;
+  const size_t mem = stage_members(h, R);
+  HIPCHECK(hipMemcpy(h.desc.IE, IE, mem * 4, hipMemcpyHostToDevice));
+  if (h.p.coding == PA_MSC && (rc = stage_put_masks(h, 0, active, R))) return rc;
+  if ((rc = stage_state(s, R, 0, 0, is, Emean))) return rc;
+  if ((rc = s.b.launch_counts()) || (rc = s.b.launch_scan())) return rc;
+  RunState st;
+  if ((rc = stage_sync(s, &st))) return rc;
+  StepOut o;
+  HIPCHECK(hipMemcpy(&o, h.desc.out + 1, sizeof o, hipMemcpyDeviceToHost));
+  if (q) *q = o.q;
+  if (lnQ) *lnQ = o.lnQ;
+  if (Roff) HIPCHECK(hipMemcpy(Roff, h.desc.Roff, mem * 4, hipMemcpyDeviceToHost));
+  if (prefix) HIPCHECK(hipMemcpy(prefix, h.desc.pre, mem * 4, hipMemcpyDeviceToHost));
+  if (Rnew) *Rnew = st.Rnew;
+  return PA_OK;
+}
+
+int pa_stage_resample(const pa_params* p, int R, const void* pop, const uint8_t* active,
+                      const uint32_t* Roff, void* pop_new, uint8_t* active_new, int* Rnew) {
+  if (!p || !pop || !Roff || !pop_new) return fail(PA_ERR_INVALID, "null argument");
+  const int w = p->coding == PA_SSC ? 1 : p->msbits;
+  if (w < 1) return fail(PA_ERR_INVALID, "msbits");
+  const size_t mem = p->coding == PA_SSC ? (size_t)R : (size_t)ceil_div(R, w) * w;
+  // the copy-count stage leaves per-1024 chunk totals for the scan; recreate them
+  std::vector<uint32_t> cblk(ceil_div((int)mem, kRedBlock), 0);
+  int64_t total = 0;
+  for (size_t n = 0; n < mem; ++n) {
+    const bool act = p->coding == PA_SSC || !active || active[n] != 0;
+    const uint32_t r = act ? Roff[n] : 0u;
+    cblk[n / kRedBlock] += r;
+    total += r;
+  }
+  if (total < 1 || total > INT32_MAX / 2) return fail(PA_ERR_INVALID, "copy counts sum");
+  StageRun s;
+  int rc = stage_open(p, R, (int)total, 0.0, 0.0, s);
+  if (rc) return rc;
+  const RunHost& h = s.h;
+  HIPCHECK(hipMemcpy(h.desc.pop[0], pop, stage_unit_bytes(h) * stage_units(h, R),
+                     hipMemcpyHostToDevice));
+  std::vector<uint32_t> ro(Roff, Roff + mem);
+  if (p->coding == PA_MSC && active)
+    for (size_t n = 0; n < mem; ++n)
+      if (!active[n]) ro[n] = 0;
+  HIPCHECK(hipMemcpy(h.desc.Roff, ro.data(), mem * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(h.desc.cblk, cblk.data(), cblk.size() * 4, hipMemcpyHostToDevice));
+  if ((rc = stage_state(s, R, 0, 0, 0, 0.0))) return rc;
+  if ((rc = s.b.launch_scan()) || (rc = s.b.launch_resample())) return rc;
+  RunState st;
+  if ((rc = stage_sync(s, &st))) return rc;
+  HIPCHECK(hipMemcpy(pop_new, h.desc.pop[1], stage_unit_bytes(h) * stage_units(h, st.Rnew),
+                     hipMemcpyDeviceToHost));
+  if (h.p.coding == PA_MSC && active_new && (rc = stage_get_masks(h, 1, active_new, st.Rnew)))
+    return rc;
+  if (Rnew) *Rnew = st.Rnew;
+  return PA_OK;
+}
+
+int pa_stage_averages(const pa_params* p, int R, const int32_t* IE, const int32_t* M,
+                      const uint8_t* active, double sums[5], int64_t* sumE, int64_t* sumM) {
+  if (!p || !IE || !M || !sums) return fail(PA_ERR_INVALID, "null argument");
+  StageRun s;
+  int rc = stage_open(p, R, R, 0.0, 0.0, s);
+  if (rc) return rc;
+  const RunHost& h = s.h;
+  const size_t mem = stage_members(h, R);
+  HIPCHECK(hipMemcpy(h.desc.IE, IE, mem * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(h.desc.M, M, mem * 4, hipMemcpyHostToDevice));
+  // the averages read the new population: buffer buf ^ 1
+  if (h.p.coding == PA_MSC && (rc = stage_put_masks(h, 1, active, R))) return rc;
+  if ((rc = stage_state(s, R, R, 0, 0, 0.0))) return rc;
+  if ((rc = s.b.launch_finish())) return rc;
+  RunState st;
+  if ((rc = stage_sync(s, &st))) return rc;
+  StepOut o;
+  HIPCHECK(hipMemcpy(&o, h.desc.out + 1, sizeof o, hipMemcpyDeviceToHost));
+  for (int k = 0; k < 5; ++k) sums[k] = o.sums[k];
+  if (sumE) *sumE = o.sumE;
+  if (sumM) *sumM = o.sumM;
+  return PA_OK;
+}
+
+}  // extern "C"
 
 // ---------------- batch ----------------
 int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out) {
