@@ -395,6 +395,7 @@ struct Group {  // runs sharing one sweep kernel variant
   size_t lds = 0;
   int bal_workers = 0;   // > 0: makespan-balanced MSC sweep with this many workers
   size_t bal_lds = 0;
+  int bal_kind = 11;     // 15: every member takes the register-cached path (fewer registers)
 };
 
 int device_cus() {
@@ -416,15 +417,19 @@ int device_cus() {
 // count so that a share always exceeds one unit (a unit is cut at most once).
 void plan_balance(Group& g, const std::vector<RunHost*>& runs) {
   g.bal_workers = 0;
-  if (g.v.coding != PA_MSC || g.v.nt > 512) return;
+  if (g.v.coding != PA_MSC || g.v.nt > 512) return;  // no balanced 1024-thread kernel
   int64_t units = 0;
   int force = 0;
+  bool fast = true;
   for (int m : g.members) {
     const RunHost& h = *runs[m];
     if (h.p.eqsweeps == 0) return;
     units += (int64_t)h.cap_units * (h.w / h.sw);
     force = h.p.balance ? h.p.balance : force;
+    const int Nh = h.N / 2;
+    fast = fast && Nh % 512 == 0 && Nh / 512 <= g.v.nt / 128 && 128 % (h.p.L / 2) == 0;
   }
+  g.bal_kind = fast ? 15 : 11;
   if (force < 0) return;
   const int cus = device_cus();
   const int64_t expect = units * 9 / 10;
@@ -433,7 +438,7 @@ void plan_balance(Group& g, const std::vector<RunHost*>& runs) {
   for (; k >= 1; --k) {
     const size_t lds = std::max(g.lds, (size_t)(160 * 1024) / k - 4096);
     int occ = 0;
-    if (pa_occupancy(11, g.v.coding, g.v.w, g.v.sw, g.v.nt, lds, &occ) != hipSuccess) return;
+    if (pa_occupancy(g.bal_kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, lds, &occ) != hipSuccess) return;
     if (occ == k) { g.bal_lds = lds; break; }
   }
   if (k < 1) return;
@@ -657,7 +662,7 @@ struct Batch {
     for (const Group& g : groups) {
       const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;
       if (g.bal_workers > 0)
-        HIPCHECK(pa_launch_step(11, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, stream,
+        HIPCHECK(pa_launch_step(g.bal_kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, stream,
                                 g.desc_d, g.offs_d, (int)g.members.size()));
       else
         HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.coding == PA_SSC ? g.v.path : g.v.sw, g.v.nt,

@@ -1002,7 +1002,9 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
 // every site word; its LCG chain starts at ran_{q*SW} = A_{q SW} ran_0 + C_{q SW}.  Slices
 // let lattices too large for LDS (L=256 x 64 bits = 512 KiB) fit, and do not change any
 // stream.  init: the freshly initialised population (buffer 0, energies only, no store).
-template <typename W, typename S, int NT>
+// FAST: the caller guarantees N/2 % 512 == 0 and nb <= NT / 128 (the register-cached path),
+// so the other paths are not compiled in and their registers do not count.
+template <typename W, typename S, int NT, bool FAST = false>
 __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const RunState* st, int g, int qs, uint32_t s_begin,
                          uint32_t s_end, bool init, bool energy, S* lat, int* accE, int* accM) {
   constexpr int w = (int)sizeof(W) * 8;
@@ -1021,7 +1023,7 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
   uint32_t JA = 1u, JC = 0u;
   lcg_jump(qs * SW, JA, JC);
   __syncthreads();
-  if ((Nh & 511) == 0) {
+  if (FAST || (Nh & 511) == 0) {
     constexpr int P = NT / kStreamsPerUnit;
     const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
     const int nb = Nh / 512;
@@ -1036,21 +1038,45 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
                                          oth[q[jj].i4], c4[jj], c8[jj]);
       }
     };
-    if (nb <= P) {
-      // at most one block per half sweep per thread: its four sites stay in registers
+    if (FAST || (nb <= P && d.r128 == 0)) {
+      // At most one block per half sweep per thread.  N/2 % 512 == 0 and nb <= 8 leave only
+      // L = 32 and 64, where L/2 divides 128: the block's four sites lie in column t % Ld2 of
+      // rows t / Ld2 + (4 part + jj) q128, so their byte offsets are o0 + 128 B jj with the
+      // horizontal delta fixed per (half, jj) and vertical wraps only above jj = 0 and below
+      // jj = 3 -- a dozen registers instead of sixteen site indices per half.
       const bool act = part < nb;
-      Site qs4[2][4];
+      constexpr int B = (int)sizeof(S);
+      const int tx = t % Ld2, ty0 = t / Ld2;
+      const int dR = (tx + 1 == Ld2 ? 1 - Ld2 : 1) * B, dL = (tx == 0 ? Ld2 - 1 : -1) * B;
+      const int dV = Ld2 * B, dN = Nh * B;
+      const int o0 = (t + 512 * part) * B, o3b = o0 + 3 * 128 * B;
+      const int up0 = o0 < dV ? o0 - dV + dN : o0 - dV;
+      const int dn3 = o3b + dV >= dN ? o3b + dV - dN : o3b + dV;
+      int d1[2][4];
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj)
-          qs4[h][jj] = site_at(t + kStreamsPerUnit * (4 * part + jj), h, d);
+        for (int jj = 0; jj < 4; ++jj) d1[h][jj] = ((ty0 + jj * d.q128 + h) & 1) ? dR : dL;
       for (uint32_t s = s_begin; s < s_end; ++s) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          if (act)
-            block4(lat + h * Nh, lat + (h ^ 1) * Nh, qs4[h],
-                   stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + part));
+          if (act) {
+            uint8_t* own = (uint8_t*)(lat + h * Nh);
+            const uint8_t* oth = (const uint8_t*)(lat + (h ^ 1) * Nh);
+            const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + part);
+            uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
+            S c4[4], c8[4];
+            lcg_masks<S, 4>(ran, T4, T8, c4, c8);
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+              const int o = o0 + jj * 128 * B;
+              const int o3 = jj == 0 ? up0 : o - dV, o4 = jj == 3 ? dn3 : o + dV;
+              const S sp = *(const S*)(own + o);
+              *(S*)(own + o) = sp ^ msc_acc<S>(sp, *(const S*)(oth + o + d1[h][jj]),
+                                               *(const S*)(oth + o), *(const S*)(oth + o3),
+                                               *(const S*)(oth + o4), c4[jj], c8[jj]);
+            }
+          }
           __syncthreads();
         }
       }
@@ -1215,8 +1241,8 @@ __device__ __forceinline__ int64_t run_units(const RunDesc& d, int NS) {
 
 // At most 128 VGPRs (4 waves per SIMD), so two 512-thread workers fit a CU; the few
 // registers this spills are cheaper than running one worker per CU (7 % at L=64).
-template <typename W, typename S, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep_msc_bal(const RunDesc* __restrict__ runs,
+template <typename W, typename S, int NT, bool FAST>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FAST ? 6 : 4))) void k_sweep_msc_bal(const RunDesc* __restrict__ runs,
                                                       int nruns, int P) {
   constexpr int SW = (int)sizeof(S) * 8;
   constexpr int NS = (int)sizeof(W) * 8 / SW;
@@ -1279,7 +1305,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
-      msc_unit<W, S, NT>(d, st, g, qs, (uint32_t)s0, (uint32_t)s1, false, !publish, lat, accE, accM);
+      msc_unit<W, S, NT, FAST>(d, st, g, qs, (uint32_t)s0, (uint32_t)s1, false, !publish, lat, accE, accM);
       if (publish) {
         __syncthreads();
         if (threadIdx.x == 0) __hip_atomic_store(flag, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1454,10 +1480,11 @@ static hipError_t launch_sweep_msc(int w, int sw, int nt, int nblocks, size_t ld
   return hipGetLastError();
 }
 
+template <bool FAST>
 static hipError_t launch_sweep_msc_bal(int w, int sw, int nt, int P, size_t lds, hipStream_t s,
                                        const RunDesc* runs, int nruns) {
 #define PA_MSCB(WT, ST, NTV) \
-  hipLaunchKernelGGL((k_sweep_msc_bal<WT, ST, NTV>), dim3(P), dim3(NTV), lds, s, runs, nruns, P)
+  hipLaunchKernelGGL((k_sweep_msc_bal<WT, ST, NTV, FAST>), dim3(P), dim3(NTV), lds, s, runs, nruns, P)
 #define PA_MSCB_NT(WT, ST)                     \
   do {                                         \
     if (nt == 512) PA_MSCB(WT, ST, 512);        \
@@ -1528,9 +1555,12 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW((k_sweep_ssc<128, 2>));
   PA_ALLOW((k_sweep_ssc<128, 3>));
 #define PA_ALLOW_MSC(WT, ST)                          \
-  PA_ALLOW(k_sweep_msc_bal<WT, ST, 128>);             \
-  PA_ALLOW(k_sweep_msc_bal<WT, ST, 256>);             \
-  PA_ALLOW(k_sweep_msc_bal<WT, ST, 512>);             \
+  PA_ALLOW((k_sweep_msc_bal<WT, ST, 128, false>));    \
+  PA_ALLOW((k_sweep_msc_bal<WT, ST, 256, false>));    \
+  PA_ALLOW((k_sweep_msc_bal<WT, ST, 512, false>));    \
+  PA_ALLOW((k_sweep_msc_bal<WT, ST, 128, true>));     \
+  PA_ALLOW((k_sweep_msc_bal<WT, ST, 256, true>));     \
+  PA_ALLOW((k_sweep_msc_bal<WT, ST, 512, true>));     \
   PA_ALLOW(k_sweep_msc<WT, ST, 128, false>);          \
   PA_ALLOW(k_sweep_msc<WT, ST, 256, false>);          \
   PA_ALLOW(k_sweep_msc<WT, ST, 512, false>);          \
@@ -1570,10 +1600,14 @@ hipError_t pa_launch_init(int coding, int w, int sw, int nt, int nblocks, size_t
 }
 
 hipError_t pa_occupancy(int kind, int coding, int w, int sw, int nt, size_t lds, int* blocks_per_cu) {
-  if (kind != 11) return hipErrorInvalidValue;
+  if (kind != 11 && kind != 15) return hipErrorInvalidValue;
   (void)coding;
-#define PA_OCC(WT, ST, NTV) \
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_sweep_msc_bal<WT, ST, NTV>, NTV, lds)
+#define PA_OCC(WT, ST, NTV)                                                                  \
+  return kind == 15                                                                          \
+             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(                                 \
+                   blocks_per_cu, k_sweep_msc_bal<WT, ST, NTV, true>, NTV, lds)              \
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(                                 \
+                   blocks_per_cu, k_sweep_msc_bal<WT, ST, NTV, false>, NTV, lds)
 #define PA_OCC_NT(WT, ST) do { if (nt == 512) PA_OCC(WT, ST, 512); else if (nt == 256) PA_OCC(WT, ST, 256); else PA_OCC(WT, ST, 128); } while (0)
   if (w == 32) {
     if (sw == 32) PA_OCC_NT(uint32_t, uint32_t);
@@ -1622,7 +1656,8 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 7: hipLaunchKernelGGL(k_fin, dim3(nblocks), dim3(64), 0, s, runs, nruns); break;
     case 8: hipLaunchKernelGGL(k_qgroup, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 9: hipLaunchKernelGGL(k_parents, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
-    case 11: return launch_sweep_msc_bal(w, sw, nt, nblocks, lds, s, runs, nruns);
+    case 11: return launch_sweep_msc_bal<false>(w, sw, nt, nblocks, lds, s, runs, nruns);
+    case 15: return launch_sweep_msc_bal<true>(w, sw, nt, nblocks, lds, s, runs, nruns);
     case 12: hipLaunchKernelGGL(k_overlap, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 13: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 0); break;
     case 14: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 1); break;
