@@ -245,7 +245,8 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   const bool aligned = (Nh % 512) == 0;
   // aligned lattices: 128 streams x P parts, P = blocks per stream per half sweep, <= 4
   const int nb = Nh / 512;
-  // unaligned MSC lattices: the Philox blocks of a stream's half sweep are dealt to P threads
+  // unaligned lattices (MSC; SSC beyond 9 sites per stream): the Philox blocks of a stream's
+  // half sweep are dealt to P threads
   const int nblk_u = ceil_div(ceil_div(Nh, 128), 4) + 1;
   const int P_u = Nh <= 512 ? 1 : nblk_u >= 8 ? 4 : nblk_u >= 4 ? 2 : 1;
   h.nt = p.threads ? p.threads : 128 * (aligned ? std::min(4, std::max(1, nb)) : P_u);
@@ -253,15 +254,19 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   // leaves room for one workgroup per CU
   if (h.nt != 128 && h.nt != 256 && h.nt != 512 && !(h.nt == 1024 && p.coding == PA_MSC))
     return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512 (or 1024 for MSC)");
-  if (!aligned && p.coding == PA_SSC) h.nt = 128;
+  if (!aligned && p.coding == PA_SSC && ceil_div(Nh, 128) <= 9) h.nt = 128;
   // SSC sweep kernel path (pa_kernels.hip, ssc_sweeps): register-cached block loop for two
-  // or three Philox blocks per thread and half sweep, register-cached stream walk for
-  // unaligned lattices with at most 4 / 9 sites per stream and half sweep
+  // or three Philox blocks per thread and half sweep (1), fixed-column block loop for more
+  // blocks when L/2 divides 128 (4), register-cached stream walk for unaligned lattices with
+  // at most 4 / 9 sites per stream and half sweep (2 / 3), blocks dealt to several threads
+  // per stream beyond that (5)
   if (p.coding == PA_SSC) {
     const int ntmax = ceil_div(Nh, 128);
     if (aligned && nb > h.nt / 128 && nb <= 3 * (h.nt / 128)) h.path = 1;
+    else if (aligned && nb > 3 * (h.nt / 128) && 128 % (p.L / 2) == 0) h.path = 4;
     else if (!aligned && ntmax <= 4) h.path = 2;
     else if (!aligned && ntmax <= 9) h.path = 3;
+    else if (!aligned) h.path = 5;
   }
   if (p.coding == PA_MSC) {
     int sw = p.slice_bits;
@@ -1228,7 +1233,7 @@ int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
   const bool aligned = (Nh % 512) == 0;
   const int nts[4] = {128, 256, 512, 1024};
   for (int nt : nts) {
-    if (p->coding == PA_SSC && (nt == 1024 || (!aligned && nt != 128))) continue;
+    if (p->coding == PA_SSC && (nt == 1024 || (!aligned && nt != 128 && Nh <= 9 * 128))) continue;
     if (p->coding == PA_SSC) {
       cands.push_back({0, nt, 0});
     } else {

@@ -834,10 +834,88 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
     ssc_walk_cached<PATH == 2 ? 4 : 9, ALWAYS1>(lat8, d, seed, sub0, nsw, tab, t, nt);
     return;
   }
+  constexpr int P = NT / kStreamsPerUnit;
+  const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
+  if constexpr (PATH == 4) {
+    // N/2 a multiple of 512 with more than 3 blocks per thread and L/2 dividing 128
+    // (L = 128, 256): every site of thread t lies in column t % Ld2 and site t + 128 k in row
+    // t / Ld2 + k q128, so the horizontal neighbour's offset is fixed per (half, jj) and the
+    // vertical neighbours wrap only above jj = 0 and below jj = 3.
+    const int Ld2 = d.Ld2, nb = Nh / 512;
+    const int tx = t % Ld2, ty0 = t / Ld2;
+    const int dR = tx + 1 == Ld2 ? 1 - Ld2 : 1, dL = tx == 0 ? Ld2 - 1 : -1;
+    int d1[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) d1[h][jj] = ((ty0 + jj * d.q128 + h) & 1) ? dR : dL;
+    for (uint32_t s = 0; s < nsw; ++s) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        int8_t* own = lat8 + h * Nh;
+        const int8_t* oth = lat8 + (h ^ 1) * Nh;
+        for (int m = part; m < nb; m += P) {
+          const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+          const int o0 = t + 512 * m;
+          Site q[4];
+          int sp[4];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const int o = o0 + jj * kStreamsPerUnit;
+            q[jj].idx = o;
+            q[jj].i1 = o + d1[h][jj];
+            q[jj].i3 = (jj == 0 && o < Ld2) ? o - Ld2 + Nh : o - Ld2;
+            q[jj].i4 = (jj == 3 && o + Ld2 >= Nh) ? o + Ld2 - Nh : o + Ld2;
+            sp[jj] = own[o];
+          }
+          ssc_flip4<ALWAYS1>(own, oth, q, sp, rb, tab);
+        }
+        __syncthreads();
+      }
+    }
+    return;
+  }
+  if constexpr (PATH == 5) {
+    // N/2 not a multiple of 512 and more than 9 sites per stream and half sweep: stream t's
+    // n_t sites take the words (2s + h) n_t + k of its stream; the Philox blocks spanning
+    // them are dealt to the P threads of the stream, a whole block flips four sites
+    // together, the first and last block of a half sweep only the sites it holds words for.
+    const int nts = t < Nh ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
+    for (uint32_t s = 0; s < nsw; ++s) {
+      for (int h = 0; h < 2; ++h) {
+        int8_t* own = lat8 + h * Nh;
+        const int8_t* oth = lat8 + (h ^ 1) * Nh;
+        const uint64_t base = (uint64_t)(2 * s + h) * nts;
+        const uint64_t b0 = base >> 2;
+        const int nblk = nts ? (int)(((base + nts - 1) >> 2) - b0 + 1) : 0;
+        for (int c = part; c < nblk; c += P) {
+          const U4 rb = stream_block(seed, sub0 + t, b0 + c);
+          const int k0 = (int)(((b0 + c) << 2) - base);
+          if (k0 >= 0 && k0 + 3 < nts) {
+            Site q[4];
+            int sp[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              q[j] = site_at(t + kStreamsPerUnit * (k0 + j), h, d);
+              sp[j] = own[q[j].idx];
+            }
+            ssc_flip4<ALWAYS1>(own, oth, q, sp, rb, tab);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int k = k0 + j;
+              if (k >= 0 && k < nts)
+                ssc_flip<ALWAYS1>(own, oth, site_at(t + kStreamsPerUnit * k, h, d), u4_word(rb, j), tab);
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    return;
+  }
   if ((Nh & 511) == 0) {
     // aligned: thread = (stream t, part); blocks m = part, part + P, ... of nb per half
-    constexpr int P = NT / kStreamsPerUnit;
-    const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
     const int nb = Nh / 512;
     if (PATH == 0 && nb <= P) {
       // at most one block per half sweep per thread: its four sites stay in registers
@@ -925,8 +1003,7 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
     }
   } else {
     // generic: one thread per stream, sites in stream order with a cached Philox block
-    const int t = threadIdx.x;
-    const bool has = t < kStreamsPerUnit && t < Nh;
+    const bool has = threadIdx.x < kStreamsPerUnit && t < Nh;
     const int nt = has ? (Nh - t + kStreamsPerUnit - 1) / kStreamsPerUnit : 0;
     U4 rb{0, 0, 0, 0};
     uint64_t cached = ~0ull;
@@ -1554,6 +1631,12 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW((k_sweep_ssc<512, 1>));
   PA_ALLOW((k_sweep_ssc<128, 2>));
   PA_ALLOW((k_sweep_ssc<128, 3>));
+  PA_ALLOW((k_sweep_ssc<128, 4>));
+  PA_ALLOW((k_sweep_ssc<256, 4>));
+  PA_ALLOW((k_sweep_ssc<512, 4>));
+  PA_ALLOW((k_sweep_ssc<128, 5>));
+  PA_ALLOW((k_sweep_ssc<256, 5>));
+  PA_ALLOW((k_sweep_ssc<512, 5>));
 #define PA_ALLOW_MSC(WT, ST)                          \
   PA_ALLOW((k_sweep_msc_bal<WT, ST, 128, false>));    \
   PA_ALLOW((k_sweep_msc_bal<WT, ST, 256, false>));    \
@@ -1638,6 +1721,18 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
           if (nt != 128) return hipErrorInvalidValue;
           if (sw == 2) hipLaunchKernelGGL((k_sweep_ssc<128, 2>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
           else hipLaunchKernelGGL((k_sweep_ssc<128, 3>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+        } else if (sw == 4 || sw == 5) {
+          if (sw == 4) {
+            if (nt == 512) hipLaunchKernelGGL((k_sweep_ssc<512, 4>), dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
+            else if (nt == 256) hipLaunchKernelGGL((k_sweep_ssc<256, 4>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+            else if (nt == 128) hipLaunchKernelGGL((k_sweep_ssc<128, 4>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+            else return hipErrorInvalidValue;
+          } else {
+            if (nt == 512) hipLaunchKernelGGL((k_sweep_ssc<512, 5>), dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
+            else if (nt == 256) hipLaunchKernelGGL((k_sweep_ssc<256, 5>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+            else if (nt == 128) hipLaunchKernelGGL((k_sweep_ssc<128, 5>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
+            else return hipErrorInvalidValue;
+          }
         } else if (sw == 1) {
           if (nt == 512) hipLaunchKernelGGL((k_sweep_ssc<512, 1>), dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
           else if (nt == 256) hipLaunchKernelGGL((k_sweep_ssc<256, 1>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);

@@ -93,7 +93,8 @@ def test_ssc_block_shapes_match_oracle(nt):
 
 
 @pytest.mark.parametrize("L,nt", [(96, 0), (96, 256), (48, 0), (16, 0), (10, 0), (6, 0), (4, 0),
-                                  (130, 0), (160, 0), (64, 128)])
+                                  (130, 0), (160, 0), (64, 128), (128, 0), (128, 128), (256, 0),
+                                  (80, 0), (144, 256), (112, 128)])
 def test_ssc_lattices_off_the_fast_path_match_oracle(L, nt):
     """SSC lattices whose half sweeps do not split into whole Philox blocks per thread
     (stream walk), or need several blocks per thread, at every block shape."""
