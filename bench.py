@@ -212,11 +212,18 @@ def batch_mix(nruns=64, seed=2026, steps=0):
     return plist
 
 
-def bench_batch(args):
-    """Batched path (one launch sequence per step for all runs, hipGraph replay) against the
-    plain path launched one run at a time, on the same runs; every batched run is checked
-    bit-exact against its plain counterpart."""
-    plist = batch_mix(args.nruns, steps=args.batch_steps)
+def small_mix(nruns=64, seed=7):
+    """Latency-bound mix: 64 small runs (L in {16,24,32}, R in [500,2000], EqSw=10, SSC/MSC32),
+    each too small to fill the device alone."""
+    import random
+    rng = random.Random(seed)
+    return [pa.make_params(L=rng.choice([16, 24, 32]), coding=rng.choice([pa.SSC, pa.MSC]),
+                           msbits=32, Rinit=rng.randint(500, 2000), eqsweeps=10, dB=DB, Bfin=1.0,
+                           seed=500 + k) for k in range(nruns)]
+
+
+def batch_vs_plain(plist):
+    """(batched wall s, plain one-at-a-time wall s, bit-exact, beta steps) for one run list."""
     t0 = time.perf_counter()
     b = pa.Batch(plist, use_graph=True)
     b.run()
@@ -228,14 +235,28 @@ def bench_batch(args):
     t_plain = time.perf_counter() - t0
     exact = all(pa.format_rows(x, 17) == pa.format_rows(y, 17) and
                 [r.spin_hash for r in x] == [r.spin_hash for r in y] for x, y in zip(brows, prows))
-    steps = sum(len(r) - 1 for r in brows)
+    return t_batch, t_plain, exact, sum(len(r) - 1 for r in brows)
+
+
+def bench_batch(args):
+    """Batched path (one launch sequence per step for all runs, hipGraph replay) against the
+    plain path launched one run at a time, on the same runs; every batched run is checked
+    bit-exact against its plain counterpart.  Also reported: the same comparison on a
+    latency-bound mix of small runs, where batching, not the kernels, sets the rate."""
+    ts, tp, ex, st = batch_vs_plain(small_mix(args.nruns))
+    small = {"runs_per_s": round(args.nruns / ts, 3), "plain_runs_per_s": round(args.nruns / tp, 3),
+             "speedup_vs_plain": round(tp / ts, 3), "bit_exact_vs_plain": ex, "beta_steps_total": st,
+             "workload": f"{args.nruns} runs, L in {{16,24,32}}, R in [500,2000], EqSw=10, "
+                         "SSC/MSC32 mixed, dB=0.005, beta 0->1"}
+    plist = batch_mix(args.nruns, steps=args.batch_steps)
+    t_batch, t_plain, exact, steps = batch_vs_plain(plist)
     return {
         "metric": "batched PA runs/s (64-run mix) vs plain one-at-a-time",
         "value": round(len(plist) / t_batch, 4), "unit": "runs/s", "n_gpus": 1,
         "higher_is_better": True, "plain_runs_per_s": round(len(plist) / t_plain, 4),
         "speedup_vs_plain": round(t_plain / t_batch, 3), "bit_exact_vs_plain": exact,
         "wall_s_batched": round(t_batch, 3), "wall_s_plain": round(t_plain, 3),
-        "beta_steps_total": steps,
+        "beta_steps_total": steps, "small_run_mix": small,
         "config": {"workload": f"{len(plist)} runs, L in {{16,24,32,48,64,96}}, R in [1000,50000], "
                                f"EqSw in {{10,50,100}}, SSC/MSC32 mixed, dB=0.005, beta 0->1"
                                + (f", truncated to {args.batch_steps} steps" if args.batch_steps else "")},
