@@ -1097,8 +1097,10 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
   const uint64_t sub0 =
       st->IS + (uint64_t)ceil_div(st->R, w) * w + (uint64_t)g * kStreamsPerUnit;
   const uint64_t seed = d.seed;
+  // slice qs starts its LCG chain SW * qs steps in; a whole-word slice (SW == w, qs == 0)
+  // starts at the stream word itself, known at compile time
   uint32_t JA = 1u, JC = 0u;
-  lcg_jump(qs * SW, JA, JC);
+  if constexpr (SW != w) lcg_jump(qs * SW, JA, JC);
   __syncthreads();
   if (FAST || (Nh & 511) == 0) {
     constexpr int P = NT / kStreamsPerUnit;
