@@ -286,9 +286,9 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     h.rs_lp = 128 / (h.rs_sites * (h.w / 8));
     if ((size_t)h.cap_units * h.rs_sites * (h.w / 8) > 150 * 1024)
       return fail(PA_ERR_INVALID, "population too large for one device resampling chunk");
-    // about one new group per thread (1024 threads per workgroup for 32-bit words, 256 for
-    // 64-bit words); the splits of one site chunk run on one XCD and re-read the chunk from
-    // that XCD's L2
+    // about one new group per thread (1024 threads per workgroup for 32-bit populations above
+    // 256 groups, 256 otherwise); the splits of one site chunk run on one XCD and re-read the
+    // chunk from that XCD's L2
     h.rs_split = std::max(1, std::min(16, ceil_div(h.cap_units, h.w == 32 ? 1024 : 256)));
   } else {
     if ((size_t)h.N + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice exceeds LDS");
@@ -460,6 +460,7 @@ struct Batch {
   int32_t* pause_d = nullptr;  // serving: last step of the tick per run
   int blocks[kKindCount] = {0, 0, 0, 0, 0};
   size_t rs_lds[2] = {0, 0};  // MSC resampling LDS for w = 32, 64
+  int rs_nt[2] = {256, 256};  // MSC resampling threads per workgroup for w = 32, 64
   bool has_w[2] = {false, false};
   std::vector<Group> groups;
   bool any_trace = false;
@@ -524,6 +525,7 @@ struct Batch {
       blocks[k] = acc;
     }
     rs_lds[0] = rs_lds[1] = 0;
+    rs_nt[0] = rs_nt[1] = 256;
     has_w[0] = has_w[1] = false;
     for (int r = 0; r < n; ++r) {
       descs[r] = runs[r]->desc;
@@ -532,6 +534,8 @@ struct Batch {
         const int wi = runs[r]->w == 32 ? 0 : 1;
         has_w[wi] = true;
         rs_lds[wi] = std::max(rs_lds[wi], runs[r]->resample_lds());
+        // 1024-thread resampling workgroups pay off for 32-bit populations above 256 groups
+        if (wi == 0 && runs[r]->cap_units > 256) rs_nt[0] = 1024;
       }
       any_trace |= runs[r]->p.trace != 0;
       any_mhr |= runs[r]->p.mhr != 0;
@@ -658,8 +662,8 @@ struct Batch {
       HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, blocks[kKResample], 0, stream, desc_d, offs_d, n));
     for (int wi = 0; wi < 2; ++wi)
       if (has_w[wi])
-        HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, 0, blocks[kKResample], rs_lds[wi], stream,
-                                desc_d, offs_d, n));
+        HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, rs_nt[wi], blocks[kKResample], rs_lds[wi],
+                                stream, desc_d, offs_d, n));
     return PA_OK;
   }
   // EqSw sweeps and energies, one launch per kernel variant

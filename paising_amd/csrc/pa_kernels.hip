@@ -691,14 +691,13 @@ __global__ __launch_bounds__(256) void k_resample(const RunDesc* __restrict__ ru
 // parents' chunks fit in LDS): the chunk of every parent group is staged in LDS once, then
 // every new word of the chunk gathers its w bits from there.  Each byte of the parent
 // population is read once and each byte of the children written once.
-// 1024 threads per workgroup for 32-bit words (one new group per thread: the large parent
-// chunk of big populations leaves room for one workgroup per CU, which 256 threads cannot
-// keep busy), 256 for 64-bit words (the 64 parent indices per thread need the registers).
-template <typename W>
-constexpr int resample_threads() { return sizeof(W) == 4 ? 1024 : 256; }
-
-template <typename W>
-__global__ __launch_bounds__(resample_threads<W>()) void k_resample_msc(const RunDesc* __restrict__ runs,
+// NT threads per workgroup, about one new group per thread: 1024 for populations of 32-bit
+// words with more than 256 groups (their large parent chunk leaves room for one workgroup
+// per CU, which 256 threads cannot keep busy), 256 otherwise (small populations; 64-bit
+// words, whose 64 parent indices per thread need the registers).  Any NT is correct: the
+// new groups of a chunk are covered by a strided loop.
+template <typename W, int NT>
+__global__ __launch_bounds__(NT) void k_resample_msc(const RunDesc* __restrict__ runs,
                                                       const int32_t* __restrict__ offs, int nruns) {
   constexpr int w = (int)sizeof(W) * 8;
   const int* off = offs + kKResample * (nruns + 1);
@@ -1623,8 +1622,9 @@ hipError_t pa_prepare_kernels(void) {
   hipError_t e = hipSuccess;
 #define PA_ALLOW(...) if (e == hipSuccess) e = allow_lds(__VA_ARGS__)
   PA_ALLOW(k_init_ssc);
-  PA_ALLOW(k_resample_msc<uint32_t>);
-  PA_ALLOW(k_resample_msc<uint64_t>);
+  PA_ALLOW((k_resample_msc<uint32_t, 256>));
+  PA_ALLOW((k_resample_msc<uint32_t, 1024>));
+  PA_ALLOW((k_resample_msc<uint64_t, 256>));
   PA_ALLOW((k_sweep_ssc<128, 0>));
   PA_ALLOW((k_sweep_ssc<256, 0>));
   PA_ALLOW((k_sweep_ssc<512, 0>));
@@ -1758,9 +1758,13 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 12: hipLaunchKernelGGL(k_overlap, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 13: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 0); break;
     case 14: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 1); break;
-    case 10:
-      if (w == 32) hipLaunchKernelGGL(k_resample_msc<uint32_t>, dim3(nblocks), dim3(resample_threads<uint32_t>()), lds, s, runs, offs, nruns);
-      else hipLaunchKernelGGL(k_resample_msc<uint64_t>, dim3(nblocks), dim3(resample_threads<uint64_t>()), lds, s, runs, offs, nruns);
+    case 10:  // nt: threads per workgroup (32-bit words: 256 or 1024)
+      if (w == 32 && nt == 1024)
+        hipLaunchKernelGGL((k_resample_msc<uint32_t, 1024>), dim3(nblocks), dim3(1024), lds, s, runs, offs, nruns);
+      else if (w == 32)
+        hipLaunchKernelGGL((k_resample_msc<uint32_t, 256>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+      else
+        hipLaunchKernelGGL((k_resample_msc<uint64_t, 256>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
       break;
     default: return hipErrorInvalidValue;
   }
