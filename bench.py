@@ -285,6 +285,13 @@ def serve_requests(n, seed=5):
 
 def bench_serve(args):
     reqs = serve_requests(args.nreq)
+    # one-time process costs (code-object load, first allocations) stay out of the latencies
+    warm = pa.Server(hbm_budget_bytes=1 << 30, max_runs=4)
+    tw = warm.submit(pa.make_params(L=16, coding=pa.MSC, msbits=32, Rinit=500, eqsweeps=5,
+                                    dB=0.05, Bfin=0.5, seed=1))
+    while not warm.done(tw):
+        warm.tick()
+    warm.close()
     policy = {"shortest": pa.SERVE_SHORTEST, "share": pa.SERVE_SHARE}[args.policy]
     srv = pa.Server(hbm_budget_bytes=200 * 10**9, max_runs=256, steps_per_tick=args.tick_steps,
                     policy=policy)
