@@ -24,7 +24,7 @@
 #include "pa_dev.h"
 #include "pa_prims.h"
 
-extern "C" hipError_t pa_launch_init(int coding, int w, int sw, int nt, int nblocks, size_t lds,
+extern "C" hipError_t pa_launch_init(int coding, int w, int sw, int nt, int half, int nblocks, size_t lds,
                                      hipStream_t s, const pa::RunDesc* runs, const int32_t* offs,
                                      int nruns, const uint64_t* is0);
 extern "C" hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nblocks,
@@ -95,7 +95,8 @@ struct RunHost {
   pa_params p{};
   int w = 1, N = 0, n_steps = 0, cap_pop = 0, cap_units = 0;
   int R0 = 0;
-  int sw = 0, nt = 0, path = 0;
+  int sw = 0, nt = 0, path = 0;  // MSC path 1: one sublattice in LDS (msc_unit_half)
+  bool half = false;
   int rs_sites = 1;
   int rs_split = 1;
   int rs_lp = 1;
@@ -163,15 +164,20 @@ TuneKey tune_key(const pa_params& p) {
 // slices for a few groups.  groups counts every group of the launch in units of this
 // lattice's sites.
 int device_cus();
-int default_slice(int w, int N, int64_t groups) {
+int default_slice(int w, int N, int64_t groups, bool half_ok, bool* half) {
   const double cus = device_cus();
   int best = 0;
   double tbest = 0;
+  *half = false;
   for (int sw = w; sw >= 8; sw /= 2) {
-    if ((size_t)N * (sw / 8) + 2048 > kLdsLimit) continue;  // the slice must fit LDS
+    // the slice must fit LDS, or (32-bit slices) one sublattice of it with the other in
+    // registers
+    const bool full = (size_t)N * (sw / 8) + 2048 <= kLdsLimit;
+    const bool hf = !full && half_ok && sw == 32;
+    if (!full && !hf) continue;
     const double c = 27.0 + 5.0 * sw;
     const double t = std::max(c, (double)groups * (w / sw) * c / cus);
-    if (!best || t < tbest) { best = sw; tbest = t; }
+    if (!best || t < tbest) { best = sw; tbest = t; *half = hf; }
   }
   return best ? best : 8;
 }
@@ -269,12 +275,26 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     else if (!aligned) h.path = 5;
   }
   if (p.coding == PA_MSC) {
+    // 32-bit slices with one sublattice in LDS and the other in registers (msc_unit_half):
+    // aligned lattices with L/2 dividing 128, at most 8 Philox blocks per thread of 1024
+    const bool half_ok = h.w >= 32 && aligned && 128 % (p.L / 2) == 0 && nb <= 64 &&
+                         (size_t)Nh * 4 + 2048 <= kLdsLimit && (!p.threads || p.threads == 1024) &&
+                         p.balance <= 0;
     int sw = p.slice_bits;
-    if (sw == 0) sw = default_slice(h.w, h.N, ceil_div(p.Rinit, h.w) + co_work / h.N);
+    bool half = false;
+    if (sw == 0) sw = default_slice(h.w, h.N, ceil_div(p.Rinit, h.w) + co_work / h.N, half_ok, &half);
     if (sw != 8 && sw != 16 && sw != 32 && sw != 64) return fail(PA_ERR_INVALID, "slice_bits");
     if (sw > h.w) sw = h.w;
-    if ((size_t)h.N * (sw / 8) + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice slice exceeds LDS");
+    if ((size_t)h.N * (sw / 8) + 1024 > kLdsLimit) {
+      if (!(sw == 32 && half_ok)) return fail(PA_ERR_INVALID, "lattice slice exceeds LDS");
+      half = true;
+    }
     h.sw = sw;
+    h.half = half;
+    if (half) {
+      h.nt = 1024;
+      h.path = 1;
+    }
     // a slice over 80 KiB leaves room for one workgroup per CU: 1024 threads keep 4 waves
     // per SIMD busy where 512 would leave 2
     if (!p.threads && (aligned ? nb >= 8 : nblk_u >= 16) && (size_t)h.N * (sw / 8) > 80 * 1024)
@@ -573,7 +593,8 @@ struct Batch {
           gis[i] = h.p.initial_sequence;
           go[(size_t)kKUnits * (m + 1) + i] = acc;
           acc += h.cap_units;
-          const size_t l = h.p.coding == PA_SSC ? (size_t)h.N : (size_t)h.N * (h.sw / 8);
+          const size_t l = h.p.coding == PA_SSC ? (size_t)h.N
+                           : (size_t)(h.half ? h.N / 2 : h.N) * (h.sw / 8);
           lds = std::max(lds, l);
         }
         go[(size_t)kKUnits * (m + 1) + m] = acc;
@@ -604,7 +625,8 @@ struct Batch {
   int launch_init() {
     for (size_t i = 0; i < groups.size(); ++i) {
       const Group& g = groups[i];
-      HIPCHECK(pa_launch_init(g.v.coding, g.v.w, g.v.sw, g.v.nt, g.units_blocks, g.lds, stream,
+      HIPCHECK(pa_launch_init(g.v.coding, g.v.w, g.v.sw, g.v.nt,
+                              g.v.coding == PA_MSC && g.v.path == 1, g.units_blocks, g.lds, stream,
                               g.desc_d, g.offs_d, (int)g.members.size(), init_is0[i]));
     }
     // MHR: the initial population enters the histogram too (SSC:500)
@@ -670,7 +692,10 @@ struct Batch {
   int launch_sweeps() {
     for (const Group& g : groups) {
       const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;
-      if (g.bal_workers > 0)
+      if (g.v.coding == PA_MSC && g.v.path == 1)
+        HIPCHECK(pa_launch_step(16, g.v.coding, g.v.w, 32, 1024, g.units_blocks * (g.v.w / 32),
+                                g.lds, stream, g.desc_d, g.offs_d, (int)g.members.size()));
+      else if (g.bal_workers > 0)
         HIPCHECK(pa_launch_step(g.bal_kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, stream,
                                 g.desc_d, g.offs_d, (int)g.members.size()));
       else
@@ -1242,7 +1267,8 @@ int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
       cands.push_back({0, nt, 0});
     } else {
       for (int sw = p->msbits; sw >= 8; sw /= 2) {
-        if ((size_t)p->L * p->L * (sw / 8) + 2048 > kLdsLimit) continue;
+        // sw = 32 may still fit with one sublattice in LDS (the plan decides)
+        if ((size_t)p->L * p->L * (sw / 8) + 2048 > kLdsLimit && sw != 32) continue;
         cands.push_back({sw, nt, -1});
         if (nt <= 512) cands.push_back({sw, nt, 1});
       }

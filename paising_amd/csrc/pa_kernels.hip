@@ -1282,6 +1282,171 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
   }
 }
 
+// msc_energy_slice for a unit whose sublattice A is in HBM (the unit's slice of every site
+// word, stride sizeof(W)) and whose sublattice B is in LDS.
+template <typename W, typename S>
+__device__ void msc_energy_half(const uint8_t* gA, const S* latB, int L, int Ld2, int Nh,
+                                int* accE, int* accM, int lane0, int32_t* IE, int32_t* Mo,
+                                size_t lane_base) {
+  constexpr int SW = (int)sizeof(S) * 8;
+  auto A = [&](int i) { return *(const S*)(gA + (size_t)i * sizeof(W)); };
+  for (int i = threadIdx.x; i < SW; i += blockDim.x) { accE[i] = 0; accM[i] = 0; }
+  __syncthreads();
+  const int nq = blockDim.x / SW > 0 ? blockDim.x / SW : 1;
+  const int lane = threadIdx.x % SW, q = threadIdx.x / SW;
+  if (q < nq) {
+    int e = 0, m = 0;
+    for (int idx = q; idx < Nh; idx += nq) {
+      const int ty = idx / Ld2, tx = idx - ty * Ld2;
+      const int iL = ty * Ld2 + (tx == 0 ? Ld2 - 1 : tx - 1);
+      const int iU = (ty == L - 1 ? 0 : ty + 1) * Ld2 + tx;
+      const S sA = A(idx), sB = latB[idx];
+      S Ai2, Bi2;
+      if (ty & 1) { Ai2 = sB; Bi2 = A(iL); } else { Ai2 = latB[iL]; Bi2 = sA; }
+      const S Ai4 = latB[iU], Bi4 = A(iU);
+      const S I1 = sA ^ Ai2, I2 = sA ^ Ai4, I3 = sB ^ Bi2, I4 = sB ^ Bi4;
+      const int c = (int)((I1 >> lane) & 1) + (int)((I2 >> lane) & 1) + (int)((I3 >> lane) & 1) +
+                    (int)((I4 >> lane) & 1);
+      e += 2 * c - 4;
+      m += 2 * ((int)((sA >> lane) & 1) + (int)((sB >> lane) & 1)) - 2;
+    }
+    atomicAdd(&accE[lane], e);
+    atomicAdd(&accM[lane], m);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < SW; i += blockDim.x) {
+    IE[lane_base + lane0 + i] = accE[i];
+    Mo[lane_base + lane0 + i] = accM[i];
+  }
+}
+
+// MSC sweeps of one unit whose 32-bit slice is too large for LDS as a whole (L = 256:
+// 256 KiB): LDS holds one sublattice (N/2 words) and the other lives in the registers of the
+// threads that update it.  Half sweep h updates the register-resident sublattice h reading
+// its neighbours from LDS (sublattice h ^ 1); then every thread swaps its sites' words with
+// the LDS copy, so the next half sweep finds h in LDS and h ^ 1 in registers.  A site's
+// thread is the one that evaluates its Philox block; the sites lie on the thread's column
+// (L/2 divides 128, N/2 % 512 == 0, at most 8 blocks per thread: the caller guarantees it).
+// Twice the replica bits per LDS word of the 16-bit slices that would fit otherwise, so
+// the per-word work is spread over twice the attempts.
+template <typename W>
+__device__ __attribute__((always_inline)) void msc_unit_half(const RunDesc& d, const RunState* st, int g, int qs, bool init,
+                              uint32_t* lat, int* accE, int* accM) {
+  using S = uint32_t;
+  constexpr int NT = 1024, P = NT / kStreamsPerUnit, MAXB = 8;
+  constexpr int w = (int)sizeof(W) * 8;
+  constexpr int SW = 32;
+  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
+  const int step = st->step;
+  const uint32_t nsw = init ? 0u : d.eqsweeps;
+  uint8_t* gbase = (uint8_t*)d.pop[init ? 0 : st->buf ^ 1] + (size_t)g * N * sizeof(W) +
+                   qs * sizeof(S);
+  const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
+  const int nb = Nh / 512;
+  const int nbt = part < nb ? (nb - part + P - 1) / P : 0;
+  __syncthreads();  // LDS reuse across units
+  S own[MAXB][4];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const int o = t + 512 * (part + P * i) + kStreamsPerUnit * jj;
+      own[i][jj] = i < nbt ? *(const S*)(gbase + (size_t)o * sizeof(W)) : 0u;
+    }
+  for (int i = threadIdx.x; i < Nh; i += NT) lat[i] = *(const S*)(gbase + (size_t)(Nh + i) * sizeof(W));
+  const uint32_t T4 = init ? 0u : (uint32_t)d.thr[3 * step + 0];
+  const uint32_t T8 = init ? 0u : (uint32_t)d.thr[3 * step + 1];
+  const uint64_t sub0 =
+      st->IS + (uint64_t)ceil_div(st->R, w) * w + (uint64_t)g * kStreamsPerUnit;
+  const uint64_t seed = d.seed;
+  uint32_t JA = 1u, JC = 0u;
+  if constexpr (SW != w) lcg_jump(qs * SW, JA, JC);
+  const int tx = t % Ld2, ty0 = t / Ld2;
+  const int dR = tx + 1 == Ld2 ? 1 - Ld2 : 1, dL = tx == 0 ? Ld2 - 1 : -1;
+  int d1[2][4];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) d1[h][jj] = ((ty0 + jj * d.q128 + h) & 1) ? dR : dL;
+  __syncthreads();
+  for (uint32_t s = 0; s < nsw; ++s) {
+    for (int h = 0; h < 2; ++h) {
+      // the sites' offsets are recomputed in every half sweep from an opaque base: held
+      // across the loop, the 32 sites' neighbour offsets would not fit the registers
+      int obase = t + 512 * part;
+      asm volatile("" : "+v"(obase));
+#pragma unroll
+      for (int i = 0; i < MAXB; ++i) {
+        if (i < nbt) {
+          const int m = part + P * i;
+          const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+          uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
+          S c4[4], c8[4];
+          lcg_masks<S, 4>(ran, T4, T8, c4, c8);
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const int o = obase + 512 * P * i + kStreamsPerUnit * jj;
+            const int o3 = (jj == 0 && o < Ld2) ? o - Ld2 + Nh : o - Ld2;
+            const int o4 = (jj == 3 && o + Ld2 >= Nh) ? o + Ld2 - Nh : o + Ld2;
+            const int dh = h ? d1[1][jj] : d1[0][jj];
+            own[i][jj] ^= msc_acc<S>(own[i][jj], lat[o + dh], lat[o], lat[o3], lat[o4], c4[jj], c8[jj]);
+          }
+        }
+      }
+      __syncthreads();
+      asm volatile("" : "+v"(obase));
+#pragma unroll
+      for (int i = 0; i < MAXB; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+          if (i < nbt) {
+            const int o = obase + 512 * P * i + kStreamsPerUnit * jj;
+            const S x = lat[o];
+            lat[o] = own[i][jj];
+            own[i][jj] = x;
+          }
+      __syncthreads();
+    }
+  }
+  // an even number of swaps: registers hold sublattice A, LDS sublattice B
+  if (!init) {
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+        if (i < nbt) {
+          const int o = t + 512 * (part + P * i) + kStreamsPerUnit * jj;
+          *(S*)(gbase + (size_t)o * sizeof(W)) = own[i][jj];
+        }
+    for (int i = threadIdx.x; i < Nh; i += NT) *(S*)(gbase + (size_t)(Nh + i) * sizeof(W)) = lat[i];
+  }
+  // energies read A back from HBM: the agent-scope fence completes this workgroup's stores and
+  // drops stale L1 lines before any wave reads them
+  __threadfence();
+  __syncthreads();
+  msc_energy_half<W, S>(gbase, lat, L, Ld2, Nh, accE, accM, qs * SW, d.IE, d.M, (size_t)g * w);
+}
+
+template <typename W, bool INIT>
+__global__ __launch_bounds__(1024) void k_sweep_msc_half(const RunDesc* __restrict__ runs,
+                                                         const int32_t* __restrict__ offs, int nruns) {
+  constexpr int w = (int)sizeof(W) * 8;
+  constexpr int NS = w / 32;
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int bu = blockIdx.x / NS;
+  const int qs = blockIdx.x % NS;
+  const int r = find_run(off, nruns, bu);
+  const RunDesc& d = runs[r];
+  const RunState* st = d.st;
+  if (!INIT && (!st->active || st->overflow)) return;
+  const int g = bu - off[r];
+  const int Rn = INIT ? st->R : st->Rnew;
+  if (g >= ceil_div(Rn, w)) return;
+  extern __shared__ __align__(16) uint8_t smem_sw[];
+  __shared__ int accE[32], accM[32];
+  msc_unit_half<W>(d, st, g, qs, INIT, (uint32_t*)smem_sw, accE, accM);
+}
+
 // MSC: one block per (group, slice), all EqSw sweeps of the step.
 template <typename W, typename S, int NT, bool INIT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep_msc(const RunDesc* __restrict__ runs,
@@ -1622,6 +1787,10 @@ hipError_t pa_prepare_kernels(void) {
   hipError_t e = hipSuccess;
 #define PA_ALLOW(...) if (e == hipSuccess) e = allow_lds(__VA_ARGS__)
   PA_ALLOW(k_init_ssc);
+  PA_ALLOW((k_sweep_msc_half<uint32_t, false>));
+  PA_ALLOW((k_sweep_msc_half<uint32_t, true>));
+  PA_ALLOW((k_sweep_msc_half<uint64_t, false>));
+  PA_ALLOW((k_sweep_msc_half<uint64_t, true>));
   PA_ALLOW((k_resample_msc<uint32_t, 256>));
   PA_ALLOW((k_resample_msc<uint32_t, 1024>));
   PA_ALLOW((k_resample_msc<uint64_t, 256>));
@@ -1667,7 +1836,7 @@ hipError_t pa_prepare_kernels(void) {
 }
 
 // Initial population of a group of runs sharing (coding, w, slice, threads).
-hipError_t pa_launch_init(int coding, int w, int sw, int nt, int nblocks, size_t lds,
+hipError_t pa_launch_init(int coding, int w, int sw, int nt, int half, int nblocks, size_t lds,
                           hipStream_t s, const RunDesc* runs, const int32_t* offs, int nruns,
                           const uint64_t* is0) {
   if (nblocks <= 0) return hipSuccess;
@@ -1681,6 +1850,13 @@ hipError_t pa_launch_init(int coding, int w, int sw, int nt, int nblocks, size_t
     hipLaunchKernelGGL(k_init_msc<uint64_t>, dim3(nblocks), dim3(128), 0, s, runs, offs, nruns, is0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (half) {
+    if (w == 32)
+      hipLaunchKernelGGL((k_sweep_msc_half<uint32_t, true>), dim3(nblocks), dim3(1024), lds, s, runs, offs, nruns);
+    else
+      hipLaunchKernelGGL((k_sweep_msc_half<uint64_t, true>), dim3(nblocks * 2), dim3(1024), lds, s, runs, offs, nruns);
+    return hipGetLastError();
+  }
   return launch_sweep_msc<true>(w, sw, nt, nblocks * (w / sw), lds, s, runs, offs, nruns);
 }
 
@@ -1755,6 +1931,12 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 9: hipLaunchKernelGGL(k_parents, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 11: return launch_sweep_msc_bal<false>(w, sw, nt, nblocks, lds, s, runs, nruns);
     case 15: return launch_sweep_msc_bal<true>(w, sw, nt, nblocks, lds, s, runs, nruns);
+    case 16:  // MSC sweeps with one sublattice in LDS (32-bit slices, 1024 threads)
+      if (w == 32)
+        hipLaunchKernelGGL((k_sweep_msc_half<uint32_t, false>), dim3(nblocks), dim3(1024), lds, s, runs, offs, nruns);
+      else
+        hipLaunchKernelGGL((k_sweep_msc_half<uint64_t, false>), dim3(nblocks), dim3(1024), lds, s, runs, offs, nruns);
+      break;
     case 12: hipLaunchKernelGGL(k_overlap, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 13: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 0); break;
     case 14: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 1); break;

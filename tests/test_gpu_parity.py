@@ -72,11 +72,13 @@ def test_balanced_batch_of_mixed_sizes():
                                        (48, 32, 32, 256), (80, 32, 32, 512), (144, 32, 16, 1024),
                                        (10, 32, 32, 256), (16, 64, 64, 512), (112, 32, 0, 0),
                                        (176, 64, 0, 0), (24, 32, 0, 0), (40, 32, 0, 0),
-                                       (192, 32, 32, 1024), (192, 32, 0, 0), (160, 32, 0, 0)])
+                                       (192, 32, 32, 1024), (192, 32, 0, 0), (160, 32, 0, 0),
+                                       (256, 32, 32, 0), (256, 64, 32, 0), (256, 64, 32, 1024)])
 def test_slices_and_block_shapes_match_oracle(L, w, sw, nt):
     """Every replica-bit slice width and block shape the autotuner may pick is bit-exact,
     on lattices whose half sweeps split into whole Philox blocks per stream and on lattices
-    whose do not (L = 10, 16, 24, 40, 48, 80, 112, 144, 176); sw = nt = 0 is the default plan."""
+    whose do not (L = 10, 16, 24, 40, 48, 80, 112, 144, 176); sw = nt = 0 is the default plan.
+    L = 256 with 32-bit slices keeps one sublattice in LDS and the other in registers."""
     args = (L, pa.MSC, w, 700, 4, 0.01, 1.0, 13, 4)
     print(f"case L={L} w={w} sw={sw} nt={nt}", flush=True)
     orows = oracle_rows(*args)

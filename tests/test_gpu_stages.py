@@ -54,7 +54,8 @@ def test_ssc_sweep_stage(L, beta, uniform, nt):
 @pytest.mark.parametrize("L,w,sw,nt,bal,beta", [(64, 32, 0, 0, 0, 0.44), (64, 32, 8, 512, -1, 1.0),
                                                 (32, 64, 16, 256, -1, 0.2), (16, 32, 0, 0, 0, 0.0),
                                                 (80, 32, 0, 0, 0, 0.6), (128, 64, 8, 1024, -1, 0.44),
-                                                (64, 32, 32, 512, 1, 0.44), (48, 64, 0, 0, 0, 0.9)])
+                                                (64, 32, 32, 512, 1, 0.44), (48, 64, 0, 0, 0, 0.9),
+                                                (256, 64, 32, 0, 0, 0.44), (256, 32, 32, 0, 0, 0.0)])
 def test_msc_sweep_stage(L, w, sw, nt, bal, beta):
     rng = np.random.default_rng(L * w + sw)
     R, IS, sweeps = 5 * w - 7, 123456789, 2
