@@ -454,19 +454,30 @@ void plan_balance(Group& g, const std::vector<RunHost*>& runs) {
     const int Nh = h.N / 2;
     fast = fast && Nh % 512 == 0 && Nh / 512 <= g.v.nt / 128 && 128 % (h.p.L / 2) == 0;
   }
-  g.bal_kind = fast ? 15 : 11;
   if (force < 0) return;
   const int cus = device_cus();
   const int64_t expect = units * 9 / 10;
   if (force == 0 && (expect < cus || expect >= 12 * (int64_t)cus)) return;
-  int k = (int)std::max<int64_t>(1, std::min<int64_t>(expect / cus, 4));
-  for (; k >= 1; --k) {
-    const size_t lds = std::max(g.lds, (size_t)(160 * 1024) / k - 4096);
-    int occ = 0;
-    if (pa_occupancy(g.bal_kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, lds, &occ) != hipSuccess) return;
-    if (occ == k) { g.bal_lds = lds; break; }
-  }
+  // the largest k <= expect / CUs (at most 4) for which exactly k workers fit a CU
+  auto workers = [&](int kind, size_t* lds_out) {
+    int k = (int)std::max<int64_t>(1, std::min<int64_t>(expect / cus, 4));
+    for (; k >= 1; --k) {
+      const size_t lds = std::max(g.lds, (size_t)(160 * 1024) / k - 4096);
+      int occ = 0;
+      if (pa_occupancy(kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, lds, &occ) != hipSuccess) return 0;
+      if (occ == k) { *lds_out = lds; return k; }
+    }
+    return 0;
+  };
+  // the instance without the other paths (fewer registers) only where it fits more workers
+  // per CU: at equal worker counts the general instance runs about 1 % faster
+  size_t lds11 = 0, lds15 = 0;
+  const int k11 = workers(11, &lds11);
+  const int k15 = fast ? workers(15, &lds15) : 0;
+  g.bal_kind = k15 > k11 ? 15 : 11;
+  const int k = std::max(k11, k15);
   if (k < 1) return;
+  g.bal_lds = g.bal_kind == 15 ? lds15 : lds11;
   g.bal_workers = k * cus;
 }
 
