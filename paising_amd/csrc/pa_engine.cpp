@@ -132,7 +132,8 @@ struct RunHost {
     return 0;
   }
   size_t resample_lds() const {
-    return p.coding == kSSC ? 0 : (size_t)cap_units * rs_sites * (w / 8);
+    // parents' chunks plus one zero chunk
+    return p.coding == kSSC ? 0 : (size_t)(cap_units + 1) * rs_sites * (w / 8);
   }
 };
 

@@ -723,10 +723,15 @@ __global__ __launch_bounds__(NT) void k_resample_msc(const RunDesc* __restrict__
   W* old = (W*)smem_rs;
   const W* src = (const W*)d.pop[buf];
   W* dst = (W*)d.pop[buf ^ 1];
+  constexpr int SM = 64 / (int)sizeof(W);  // largest chunk: 64 bytes per group
+  // whole 64-byte chunks (the common case) take a branch-free gather with vector LDS reads;
+  // lanes beyond R_new read a zero chunk staged after the parents'
+  const bool whole = S == SM && ns == SM;
   for (int i = threadIdx.x; i < rgo * S; i += blockDim.x) {
     const int g = i / S, s = i - g * S;
     if (s < ns) old[i] = src[(size_t)g * N + s0 + s];
   }
+  if (whole && threadIdx.x < SM) old[rgo * SM + threadIdx.x] = 0;
   __syncthreads();
   for (int j = js * blockDim.x + threadIdx.x; j < rgn; j += d.rs_split * blockDim.x) {
     // parents of the w lanes of new group j: independent loads issued together
@@ -736,7 +741,30 @@ __global__ __launch_bounds__(NT) void k_resample_msc(const RunDesc* __restrict__
       const int64_t n = (int64_t)k * rgn + j;
       pl[k] = (n < Rn) ? d.par[n] : -1;
     }
-    constexpr int SM = 64 / (int)sizeof(W);  // largest chunk: 64 bytes per group
+    if (whole) {
+      W acc[SM];
+#pragma unroll
+      for (int s = 0; s < SM; ++s) acc[s] = 0;
+#pragma unroll
+      for (int k = 0; k < w; ++k) {
+        const int p = pl[k];
+        const unsigned pg = p < 0 ? (unsigned)rgo : (unsigned)p / w;
+        const unsigned pb = p < 0 ? 0u : (unsigned)p % w;
+        const W* o = old + pg * SM;
+#pragma unroll
+        for (int s = 0; s < SM; ++s) acc[s] |= ((o[s] >> pb) & (W)1) << k;
+      }
+      W* out = dst + (size_t)j * N + s0;
+#pragma unroll
+      for (int s = 0; s < SM; ++s) out[s] = acc[s];
+      if (c == 0) {
+        uint64_t m = 0;
+        for (int k = 0; k < w; ++k)
+          if ((int64_t)k * rgn + j < Rn) m |= 1ull << k;
+        d.amask[buf ^ 1][j] = m;
+      }
+      continue;
+    }
     W acc[SM];
 #pragma unroll
     for (int s = 0; s < SM; ++s) acc[s] = 0;
