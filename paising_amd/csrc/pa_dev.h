@@ -29,6 +29,8 @@ struct RunState {
   int32_t active;    // 0 once the run has finished its schedule
   int32_t err;       // device-side error bits (1: balanced-sweep hand-over wait timed out)
   int32_t pause;     // serving: the run stops (active = 0) after this step; 0 = never
+  int32_t qdone;     // k_qgroup blocks finished in the current launch (last-block election)
+  int32_t avdone;    // k_avg blocks finished in the current launch
 };
 
 // Per-step record written by the finalize kernel (row i of the results file + internals).

@@ -648,7 +648,8 @@ struct Batch {
     return PA_OK;
   }
 
-  // One beta step of every active run (8 launches + one sweep launch per variant).
+  // One beta step of every active run (5 launches, 6 with single-spin runs, + one sweep launch
+  // per variant).
   int timing_event(hipEvent_t* e) {
     if (tev_used == tev.size()) {
       hipEvent_t x;
@@ -679,7 +680,8 @@ struct Batch {
     const int n = (int)runs.size();
     if (has_w[0] || has_w[1])
       HIPCHECK(pa_launch_step(8, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
-    HIPCHECK(pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n));
+    if (has_ssc())
+      HIPCHECK(pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n));
     HIPCHECK(pa_launch_step(1, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     return PA_OK;
   }
@@ -687,7 +689,6 @@ struct Batch {
   int launch_scan() {
     const int n = (int)runs.size();
     HIPCHECK(pa_launch_step(2, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
-    HIPCHECK(pa_launch_step(9, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     return PA_OK;
   }
   int launch_resample() {
@@ -717,13 +718,13 @@ struct Batch {
     }
     return PA_OK;
   }
-  // averages (CalcAverages), optional hashes and histograms, then the step record (k_fin)
+  // optional hashes and histograms, then the averages (CalcAverages), whose last block per
+  // run writes the step record and advances the run
   int launch_finish() {
     const int n = (int)runs.size();
-    HIPCHECK(pa_launch_step(5, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n));
     if (any_trace) HIPCHECK(pa_launch_step(6, 0, 0, 0, 0, blocks[kKUnits], 0, stream, desc_d, offs_d, n));
     if (any_mhr) HIPCHECK(pa_launch_step(13, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
-    HIPCHECK(pa_launch_step(7, 0, 0, 0, 0, n, 0, stream, desc_d, offs_d, n));
+    HIPCHECK(pa_launch_step(5, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n));
     return PA_OK;
   }
 
@@ -787,7 +788,8 @@ struct Batch {
       hipError_t e = hipMemcpyAsync((void*)(h.desc.dbeta + i), &stage_db, 8, hipMemcpyHostToDevice, stream);
       if (e == hipSuccess && h.p.coding == PA_MSC)
         e = pa_launch_step(8, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n);
-      if (e == hipSuccess) e = pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n);
+      if (e == hipSuccess && h.p.coding == PA_SSC)
+        e = pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n);
       if (e == hipSuccess) e = pa_launch_step(12, 0, 0, 0, 0, blocks[kKAvg], 0, stream, desc_d, offs_d, n);
       if (e == hipSuccess)
         e = hipMemcpyAsync(stage_part.data(), h.desc.avblk, stage_part.size() * 8, hipMemcpyDeviceToHost, stream);

@@ -1,19 +1,21 @@
 // gfx950 kernels of the population-annealing engine.
 //
 // One temperature step of one or many runs is the launch sequence
-//   k_qpart -> k_tau -> k_scan -> k_resample -> k_sweep_{ssc,msc} -> k_avg -> k_fin
+//   k_qgroup (MSC) / k_qpart (SSC) -> k_tau -> k_scan -> k_resample -> k_sweep_{ssc,msc} -> k_avg
+// Where a stage needs every block of the previous one (the q sums, the step record), the
+// last block of a run to finish does it (last_block), which saves a launch per stage.
 // Every kernel maps blockIdx.x to (run, local block) through a prefix table, so the same
 // launches serve a single run (plain path) and a batch of independent runs.  All
 // population-size-dependent decisions read the device RunState, never the host.
 //
 // Reference counterparts (PAisingSSC.cu = SSC, PAisingMSC.cu = MSC):
 //   k_init_*   ReplicaInit + energyKer       SSC:150-157,195-215 / MSC:169-188,268-317
-//   k_qpart    QKer part 0 (and 1 for MSC)   SSC:217-224 / MSC:319-335
+//   k_qpart / k_qgroup  QKer parts 0 and 1   SSC:217-224 / MSC:319-335
 //   k_tau      QKer final part, host lnQ, CalcTauKer   SSC:225-254,533 / MSC:336-360,696
-//   k_scan     CalcParSum                    SSC:256-270 / MSC:362-395
+//   k_scan     CalcParSum and the parent of every child  SSC:256-270 / MSC:362-395
 //   k_resample resampleKer                   SSC:272-280 / MSC:397-435
 //   k_sweep_*  checkKerALL + energyKer       SSC:159-215 / MSC:190-317
-//   k_avg/k_fin CalcAverages + host thermodynamics bookkeeping  SSC:282-292,614-632
+//   k_avg      CalcAverages + host thermodynamics bookkeeping  SSC:282-292,614-632
 
 #include <hip/hip_runtime.h>
 
@@ -68,6 +70,57 @@ __device__ T block_sum_int(T v, T* sh) {
   if (threadIdx.x == 0)
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
   return r;
+}
+
+// K tree sums at once, each in ref_block_sum's order (sh: 32 K doubles); valid in thread 0.
+template <int K>
+__device__ void ref_block_sums(double (&v)[K], double* sh) {
+  const int lane = threadIdx.x & 31, wid = threadIdx.x >> 5;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = warp32_tree(v[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k * 32 + wid] = v[k];
+  __syncthreads();
+  const int nw = blockDim.x >> 5;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = (threadIdx.x < (unsigned)nw) ? sh[k * 32 + lane] : 0.0;
+  if (wid == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = warp32_tree(v[k]);
+}
+
+// Partials that the last block of the same launch reads (last_block) are written and read
+// with device-scope relaxed atomics: these go to the device-coherent level themselves, so
+// neither side needs the L2 write-back or invalidation of an agent-scope fence (one per
+// block of every run of a batch would cost more than the launch they save).
+template <typename T>
+__device__ __forceinline__ void put_dev(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T get_dev(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Last-block election.  Every participating block of a run calls this once, with all its
+// threads, after writing its partials with put_dev; it returns true in the one block that
+// arrives last (whose get_dev reads then see every other block's partials) and re-arms the
+// counter for the next launch.  Each thread waits for its own stores to complete before the
+// block's arrival is counted.  Blocks that do not participate must decide so from state
+// that the last block leaves as it is.
+__device__ bool last_block(int32_t* ctr, int nblocks) {
+  __shared__ int last_sh;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int prev = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    last_sh = prev == nblocks - 1;
+    if (prev == nblocks - 1) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  return last_sh != 0;
 }
 
 // ---- geometry --------------------------------------------------------------------------
@@ -410,33 +463,38 @@ __global__ __launch_bounds__(1024) void k_qgroup(const RunDesc* __restrict__ run
     const double hi = __shfl_down(f, 32, 64);
     f = f + hi;
   }
-  if (lane == 0 && g < rg) d.gsum[g] = f;
+  if (lane == 0 && g < rg) put_dev(d.gsum + g, f);
+  // QKer part 1 (MSC:330-335) in the run's last block: tree sums of 1024 group sums each
+  if (!last_block(&st->qdone, ceil_div(rg * w, kRB))) return;
+  __shared__ double sh[32];
+  for (int b = 0; b * kRB < rg; ++b) {
+    const int gi = b * kRB + threadIdx.x;
+    const double s = ref_block_sum(gi < rg ? get_dev(d.gsum + gi) : 0.0, sh);
+    if (threadIdx.x == 0) d.qblk[b] = s;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    StepOut& o = d.out[st->step];
+    o.roff_hash = 0;
+    o.spin_hash = 0;
+  }
 }
 
-// SSC (QKer part 0, SSC:220-224): one block per 1024 replicas.
-// MSC (QKer part 1, MSC:330-335): one block per 1024 group sums.
+// SSC (QKer part 0, SSC:220-224): one block per 1024 replicas (MSC populations: k_qgroup).
 __global__ __launch_bounds__(1024) void k_qpart(const RunDesc* __restrict__ runs,
                                                 const int32_t* __restrict__ offs, int nruns) {
   const int* off = offs + kKGroupChunks * (nruns + 1);
   const int r = find_run(off, nruns, blockIdx.x);
   const RunDesc& d = runs[r];
   RunState* st = d.st;
-  if (!st->active || st->overflow) return;
+  if (d.coding != kSSC || !st->active || st->overflow) return;
   const int c = blockIdx.x - off[r];
   const int R = st->R, step = st->step;
   const double dB = d.dbeta[step], Emean = st->Emean;
   __shared__ double sh[32];
-  double f = 0.0;
-  if (d.coding == kSSC) {
-    if (c >= ceil_div(R, kRB)) return;
-    const int idx = c * kRB + threadIdx.x;
-    f = (idx < R) ? exp_fp64(-dB * ((double)d.IE[idx] - Emean)) : 0.0;
-  } else {
-    const int rg = ceil_div(R, d.w);
-    if (c >= ceil_div(rg, kRB)) return;
-    const int g = c * kRB + threadIdx.x;
-    f = (g < rg) ? d.gsum[g] : 0.0;
-  }
+  if (c >= ceil_div(R, kRB)) return;
+  const int idx = c * kRB + threadIdx.x;
+  const double f = (idx < R) ? exp_fp64(-dB * ((double)d.IE[idx] - Emean)) : 0.0;
   const double s = ref_block_sum(f, sh);
   if (threadIdx.x == 0) {
     d.qblk[c] = s;
@@ -576,7 +634,7 @@ __global__ __launch_bounds__(1024) void k_overlap(const RunDesc* __restrict__ ru
 // k_hist: adds the population's energies to the run's MHR histogram, bin (2N + E) / 4
 // (UpdateShistE, SSC:296-302 / MSC:454-460).  init = 1: the initial population (buffer
 // st->buf, R members); otherwise the population just equilibrated by this step (buffer
-// st->buf ^ 1, Rnew members; launched before k_fin).  Integer atomics: order-free.
+// st->buf ^ 1, Rnew members; launched before k_avg, whose last block ends the step).  Integer atomics: order-free.
 // ============================================================================================
 __global__ __launch_bounds__(1024) void k_hist(const RunDesc* __restrict__ runs,
                                                const int32_t* __restrict__ offs, int nruns,
@@ -596,7 +654,9 @@ __global__ __launch_bounds__(1024) void k_hist(const RunDesc* __restrict__ runs,
 }
 
 // ============================================================================================
-// k_scan: exclusive prefix of Roff in population order (CalcParSum), R_i, capacity check.
+// k_scan: exclusive prefix of Roff in population order (CalcParSum), the parent of every
+// child (children of member p are pre[p] .. pre[p] + Roff[p] - 1, SSC:256-280 /
+// MSC:362-435), R_i, capacity check.
 // ============================================================================================
 __global__ __launch_bounds__(1024) void k_scan(const RunDesc* __restrict__ runs,
                                                const int32_t* __restrict__ offs, int nruns) {
@@ -634,31 +694,19 @@ __global__ __launch_bounds__(1024) void k_scan(const RunDesc* __restrict__ runs,
   uint32_t wpre = 0;
   for (int i = 0; i < wv; ++i) wpre += wsum[i];
   const uint32_t excl = base_sh + wpre + x - v;
-  if (n < nel) d.pre[n] = excl;
+  if (n < nel) {
+    d.pre[n] = excl;
+    // children excl .. excl + v - 1 of member n, in parent order (resampleKer's parent
+    // search); past the capacity the run overflows and stops, so those are not written
+    const uint32_t cap = (uint32_t)d.cap_units * (uint32_t)(d.coding == kSSC ? 1 : w);
+    for (uint32_t i = 0; i < v && excl + i < cap; ++i) d.par[excl + i] = n;
+  }
   if (c == nch - 1 && threadIdx.x == blockDim.x - 1) {
     const uint32_t Rn = excl + v;
     st->Rnew = (int)Rn;
     const int units_needed = (d.coding == kSSC) ? (int)Rn : ceil_div((int)Rn, w);
     if (Rn == 0 || units_needed > d.cap_units) st->overflow = 1;
   }
-}
-
-// Parent of every child: children of parent p are pre[p] .. pre[p] + Roff[p] - 1, in
-// parent order (CalcParSum + resampleKer, SSC:256-280 / MSC:362-435).
-__global__ __launch_bounds__(1024) void k_parents(const RunDesc* __restrict__ runs,
-                                                  const int32_t* __restrict__ offs, int nruns) {
-  const int* off = offs + kKChunks * (nruns + 1);
-  const int r = find_run(off, nruns, blockIdx.x);
-  const RunDesc& d = runs[r];
-  RunState* st = d.st;
-  if (!st->active || st->overflow) return;
-  const int c = blockIdx.x - off[r];
-  const int R = st->R, w = d.w;
-  const int nel = (d.coding == kSSC) ? R : ceil_div(R, w) * w;
-  const int p = c * kRB + threadIdx.x;
-  if (p >= nel) return;
-  const uint32_t k = d.Roff[p], b = d.pre[p];
-  for (uint32_t i = 0; i < k; ++i) d.par[b + i] = p;
 }
 
 // SSC (resampleKer SSC:272-280): child j receives its parent's N spins; all N sites are
@@ -1589,8 +1637,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FAST ? 6 : 4
 // ============================================================================================
 // Averages (CalcAverages SSC:282-292 / MSC:437-450): block tree sums of E, E^2, |M|, M^2,
 // M^4 over 1024 replicas (SSC) or over 1024 groups for one lane (MSC: block = (chunk, lane)),
-// plus integer sums.  Block partials are combined in launch order by k_fin.
+// plus integer sums.  The run's last block adds the block partials in launch order and
+// writes the step record (finish_step).
 // ============================================================================================
+__device__ void finish_step(const RunDesc& d, RunState* st, int nblk);
+
 __global__ __launch_bounds__(1024) void k_avg(const RunDesc* __restrict__ runs,
                                               const int32_t* __restrict__ offs, int nruns) {
   const int* off = offs + kKAvg * (nruns + 1);
@@ -1600,18 +1651,20 @@ __global__ __launch_bounds__(1024) void k_avg(const RunDesc* __restrict__ runs,
   if (!st->active || st->overflow) return;
   const int c = blockIdx.x - off[r];
   const int Rn = st->Rnew, w = d.w, buf = st->buf ^ 1;
-  int nb, b, by;
+  int nb, b, by, nblk;
   if (d.coding == kSSC) {
     nb = ceil_div(Rn, kRB);
+    nblk = nb;
     if (c >= nb) return;
     b = c; by = 0;
   } else {
     nb = ceil_div(ceil_div(Rn, w), kRB);
-    if (c >= nb * w) return;
+    nblk = nb * w;
+    if (c >= nblk) return;
     by = c / nb; b = c % nb;
   }
-  __shared__ double sh[32];
-  __shared__ int64_t shl[16];
+  __shared__ double sh[5 * 32];
+  __shared__ int64_t shl[2][16];
   const int idx = b * kRB + threadIdx.x;
   double e = 0.0, m = 0.0;
   int64_t ei = 0, mi = 0;
@@ -1625,17 +1678,20 @@ __global__ __launch_bounds__(1024) void k_avg(const RunDesc* __restrict__ runs,
     }
   }
   if (m < 0) m = -m;
-  const double e2 = e * e, m2 = m * m, m4 = m2 * m2;
-  double* o = d.avblk + (size_t)c * 5;
-  double s;
-  s = ref_block_sum(e, sh);  if (threadIdx.x == 0) o[0] = s;  __syncthreads();
-  s = ref_block_sum(e2, sh); if (threadIdx.x == 0) o[1] = s;  __syncthreads();
-  s = ref_block_sum(m, sh);  if (threadIdx.x == 0) o[2] = s;  __syncthreads();
-  s = ref_block_sum(m2, sh); if (threadIdx.x == 0) o[3] = s;  __syncthreads();
-  s = ref_block_sum(m4, sh); if (threadIdx.x == 0) o[4] = s;  __syncthreads();
-  const int64_t se = block_sum_int<int64_t>(ei, shl);
-  const int64_t sm = block_sum_int<int64_t>(mi, shl);
-  if (threadIdx.x == 0) { d.iblk[2 * c] = se; d.iblk[2 * c + 1] = sm; }
+  double v[5] = {e, e * e, m, m * m, 0.0};
+  v[4] = v[3] * v[3];
+  ref_block_sums<5>(v, sh);
+  const int64_t se = block_sum_int<int64_t>(ei, shl[0]);
+  const int64_t sm = block_sum_int<int64_t>(mi, shl[1]);
+  if (threadIdx.x == 0) {
+    double* o = d.avblk + (size_t)c * 5;
+    for (int q = 0; q < 5; ++q) put_dev(o + q, v[q]);
+    put_dev(d.iblk + 2 * c, se);
+    put_dev(d.iblk + 2 * c + 1, sm);
+  }
+  // the state finish_step advances (R, buf, step, active) decides nothing above but active,
+  // which retires late blocks as well
+  if (last_block(&st->avdone, nblk)) finish_step(d, st, nblk);
 }
 
 // Spin-configuration hash of the new population (trace mode only): sum over 8-byte chunks
@@ -1667,26 +1723,32 @@ __global__ __launch_bounds__(256) void k_hash(const RunDesc* __restrict__ runs,
     atomicAdd((unsigned long long*)&d.out[st->step].spin_hash, (unsigned long long)t);
 }
 
-// Step bookkeeping (SSC:620-632 device part): combine the average partials in launch order,
+// Step bookkeeping (SSC:620-632 device part): add the average partials in launch order,
 // write the step record, advance R, IS (tau draws + sweep streams), Emean and the step.
-__global__ void k_fin(const RunDesc* __restrict__ runs, int nruns) {
-  const int r = blockIdx.x;
-  if (r >= nruns || threadIdx.x != 0) return;
-  const RunDesc& d = runs[r];
-  RunState* st = d.st;
-  if (!st->active || st->overflow) return;
-  const int step = st->step, R = st->R, Rn = st->Rnew, w = d.w;
-  StepOut& o = d.out[step];
+// Run by the last k_avg block of the run; the partials pass through LDS 256 blocks at a time
+// and thread 0 adds them.
+__device__ void finish_step(const RunDesc& d, RunState* st, int nblk) {
+  constexpr int kC = 256;
+  __shared__ double fs[kC * 5];
+  __shared__ int64_t fi[kC * 2];
   double sums[5] = {0, 0, 0, 0, 0};
   int64_t se = 0, sm = 0;
-  int nblk;
-  if (d.coding == kSSC) nblk = ceil_div(Rn, kRB);
-  else nblk = ceil_div(ceil_div(Rn, w), kRB) * w;
-  for (int c = 0; c < nblk; ++c) {
-    for (int q = 0; q < 5; ++q) sums[q] += d.avblk[(size_t)c * 5 + q];
-    se += d.iblk[2 * c];
-    sm += d.iblk[2 * c + 1];
+  for (int c0 = 0; c0 < nblk; c0 += kC) {
+    const int m = min(kC, nblk - c0);
+    for (int i = threadIdx.x; i < m * 5; i += blockDim.x) fs[i] = get_dev(d.avblk + (size_t)c0 * 5 + i);
+    for (int i = threadIdx.x; i < m * 2; i += blockDim.x) fi[i] = get_dev(d.iblk + (size_t)c0 * 2 + i);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int c = 0; c < m; ++c) {
+        for (int q = 0; q < 5; ++q) sums[q] += fs[c * 5 + q];
+        se += fi[2 * c];
+        sm += fi[2 * c + 1];
+      }
+    __syncthreads();
   }
+  if (threadIdx.x != 0) return;
+  const int step = st->step, R = st->R, Rn = st->Rnew, w = d.w;
+  StepOut& o = d.out[step];
   for (int q = 0; q < 5; ++q) o.sums[q] = sums[q];
   o.sumE = se;
   o.sumM = sm;
@@ -1954,9 +2016,7 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
       break;
     case 5: hipLaunchKernelGGL(k_avg, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 6: hipLaunchKernelGGL(k_hash, dim3(nblocks), dim3(256), 0, s, runs, offs, nruns); break;
-    case 7: hipLaunchKernelGGL(k_fin, dim3(nblocks), dim3(64), 0, s, runs, nruns); break;
     case 8: hipLaunchKernelGGL(k_qgroup, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
-    case 9: hipLaunchKernelGGL(k_parents, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns); break;
     case 11: return launch_sweep_msc_bal<false>(w, sw, nt, nblocks, lds, s, runs, nruns);
     case 15: return launch_sweep_msc_bal<true>(w, sw, nt, nblocks, lds, s, runs, nruns);
     case 16:  // MSC sweeps with one sublattice in LDS (32-bit slices, 1024 threads)
