@@ -495,6 +495,11 @@ struct Batch {
   int rs_nt[2] = {256, 256};  // MSC resampling threads per workgroup for w = 32, 64
   bool has_w[2] = {false, false};
   std::vector<Group> groups;
+  struct RsGroup {  // the runs of one resampling kernel (SSC, MSC32, MSC64)
+    RunDesc* desc_d = nullptr;
+    int32_t* offs_d = nullptr;
+    int m = 0, blocks = 0;
+  } rs_groups[3];
   bool any_trace = false;
   bool any_mhr = false;
   bool use_graph = false;
@@ -519,6 +524,7 @@ struct Batch {
     for (void* p : table_allocs) (void)hipFree(p);
     table_allocs.clear();
     groups.clear();
+    for (RsGroup& g : rs_groups) g = RsGroup{};
     init_is0.clear();
   }
   void release() {
@@ -624,6 +630,34 @@ struct Batch {
         init_is0.push_back(gis_d);
       }
     }
+    // resampling tables per kernel (SSC, MSC32, MSC64): each launch covers only the runs
+    // its kernel serves, not the blocks of the other codings' runs
+    for (int k = 0; k < 3; ++k) {
+      RsGroup& g = rs_groups[k];
+      g = RsGroup{};
+      std::vector<RunDesc> gd;
+      for (int r = 0; r < n; ++r) {
+        const RunHost& h = *runs[r];
+        const int kr = h.p.coding == PA_SSC ? 0 : h.w == 32 ? 1 : 2;
+        if (kr == k) gd.push_back(h.desc);
+      }
+      g.m = (int)gd.size();
+      if (!g.m) continue;
+      std::vector<int32_t> go((size_t)kKindCount * (g.m + 1), 0);
+      int acc = 0, i = 0;
+      for (int r = 0; r < n; ++r) {
+        const RunHost& h = *runs[r];
+        if ((h.p.coding == PA_SSC ? 0 : h.w == 32 ? 1 : 2) != k) continue;
+        go[(size_t)kKResample * (g.m + 1) + i++] = acc;
+        acc += h.blocks(kKResample);
+      }
+      go[(size_t)kKResample * (g.m + 1) + g.m] = acc;
+      g.blocks = acc;
+      if ((rc = alloc_table((void**)&g.desc_d, g.m * sizeof(RunDesc)))) return rc;
+      if ((rc = alloc_table((void**)&g.offs_d, go.size() * 4))) return rc;
+      HIPCHECK(hipMemcpy(g.desc_d, gd.data(), g.m * sizeof(RunDesc), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(g.offs_d, go.data(), go.size() * 4, hipMemcpyHostToDevice));
+    }
     return PA_OK;
   }
   std::vector<uint64_t*> init_is0;
@@ -692,13 +726,14 @@ struct Batch {
     return PA_OK;
   }
   int launch_resample() {
-    const int n = (int)runs.size();
-    if (has_w[0] || has_w[1] ? has_ssc() : true)
-      HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, blocks[kKResample], 0, stream, desc_d, offs_d, n));
-    for (int wi = 0; wi < 2; ++wi)
-      if (has_w[wi])
-        HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, rs_nt[wi], blocks[kKResample], rs_lds[wi],
-                                stream, desc_d, offs_d, n));
+    const RsGroup& gs = rs_groups[0];
+    if (gs.m) HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, gs.blocks, 0, stream, gs.desc_d, gs.offs_d, gs.m));
+    for (int wi = 0; wi < 2; ++wi) {
+      const RsGroup& g = rs_groups[1 + wi];
+      if (g.m)
+        HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, rs_nt[wi], g.blocks, rs_lds[wi], stream,
+                                g.desc_d, g.offs_d, g.m));
+    }
     return PA_OK;
   }
   // EqSw sweeps and energies, one launch per kernel variant
