@@ -533,6 +533,11 @@ struct Batch {
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
     tev.clear();
     tev_used = 0;
+    for (hipStream_t x : side) (void)hipStreamDestroy(x);
+    for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
+    side.clear();
+    join_ev.clear();
+    if (fork_ev) { (void)hipEventDestroy(fork_ev); fork_ev = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
   }
 
@@ -658,7 +663,7 @@ struct Batch {
       HIPCHECK(hipMemcpy(g.desc_d, gd.data(), g.m * sizeof(RunDesc), hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(g.offs_d, go.data(), go.size() * 4, hipMemcpyHostToDevice));
     }
-    return PA_OK;
+    return ensure_side_streams();
   }
   std::vector<uint64_t*> init_is0;
 
@@ -736,20 +741,57 @@ struct Batch {
     }
     return PA_OK;
   }
-  // EqSw sweeps and energies, one launch per kernel variant
+  // EqSw sweeps and energies, one launch per kernel variant.  With several variants the
+  // launches after the first go to side streams forked from the batch stream and joined back
+  // into it (parallel branches of a captured step), so a variant's tail and the small
+  // variants overlap the others instead of queueing behind them.  The variants' runs are
+  // disjoint.  A balanced launch whose workers start late only waits longer: each waits on a
+  // worker of higher index, and workgroups of one launch are dispatched in index order.
+  int launch_group(const Group& g, hipStream_t s) {
+    const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;
+    if (g.v.coding == PA_MSC && g.v.path == 1)
+      HIPCHECK(pa_launch_step(16, g.v.coding, g.v.w, 32, 1024, g.units_blocks * (g.v.w / 32),
+                              g.lds, s, g.desc_d, g.offs_d, (int)g.members.size()));
+    else if (g.bal_workers > 0)
+      HIPCHECK(pa_launch_step(g.bal_kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, s,
+                              g.desc_d, g.offs_d, (int)g.members.size()));
+    else
+      HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.coding == PA_SSC ? g.v.path : g.v.sw, g.v.nt,
+                              g.units_blocks * ns, g.lds, s, g.desc_d, g.offs_d, (int)g.members.size()));
+    return PA_OK;
+  }
   int launch_sweeps() {
-    for (const Group& g : groups) {
-      const int ns = g.v.coding == PA_SSC ? 1 : g.v.w / g.v.sw;
-      if (g.v.coding == PA_MSC && g.v.path == 1)
-        HIPCHECK(pa_launch_step(16, g.v.coding, g.v.w, 32, 1024, g.units_blocks * (g.v.w / 32),
-                                g.lds, stream, g.desc_d, g.offs_d, (int)g.members.size()));
-      else if (g.bal_workers > 0)
-        HIPCHECK(pa_launch_step(g.bal_kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, stream,
-                                g.desc_d, g.offs_d, (int)g.members.size()));
-      else
-        HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.coding == PA_SSC ? g.v.path : g.v.sw, g.v.nt,
-                                g.units_blocks * ns, g.lds, stream,
-                                g.desc_d, g.offs_d, (int)g.members.size()));
+    const size_t ng = groups.size();
+    int rc;
+    if (ng < 2 || side.size() + 1 < ng) {
+      for (const Group& g : groups)
+        if ((rc = launch_group(g, stream))) return rc;
+      return PA_OK;
+    }
+    HIPCHECK(hipEventRecord(fork_ev, stream));
+    for (size_t i = 1; i < ng; ++i) {
+      HIPCHECK(hipStreamWaitEvent(side[i - 1], fork_ev, 0));
+      if ((rc = launch_group(groups[i], side[i - 1]))) return rc;
+      HIPCHECK(hipEventRecord(join_ev[i - 1], side[i - 1]));
+    }
+    if ((rc = launch_group(groups[0], stream))) return rc;
+    for (size_t i = 1; i < ng; ++i) HIPCHECK(hipStreamWaitEvent(stream, join_ev[i - 1], 0));
+    return PA_OK;
+  }
+  // side streams and fork/join events for the sweep variants (outside any capture)
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join_ev;
+  hipEvent_t fork_ev = nullptr;
+  int ensure_side_streams() {
+    const size_t need = groups.size() > 1 ? groups.size() - 1 : 0;
+    if (need && !fork_ev) HIPCHECK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    while (side.size() < need) {
+      hipStream_t x;
+      HIPCHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+      side.push_back(x);
+      hipEvent_t e;
+      HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      join_ev.push_back(e);
     }
     return PA_OK;
   }
