@@ -307,10 +307,13 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     h.rs_lp = 128 / (h.rs_sites * (h.w / 8));
     if ((size_t)h.cap_units * h.rs_sites * (h.w / 8) > 150 * 1024)
       return fail(PA_ERR_INVALID, "population too large for one device resampling chunk");
-    // about one new group per thread (1024 threads per workgroup for 32-bit populations above
-    // 256 groups, 256 otherwise); the splits of one site chunk run on one XCD and re-read the
-    // chunk from that XCD's L2
-    h.rs_split = std::max(1, std::min(16, ceil_div(h.cap_units, h.w == 32 ? 1024 : 256)));
+    // splits of a site chunk share its new groups (1024 threads per workgroup for 32-bit
+    // populations above 256 groups, 256 otherwise) and run on one XCD, but each stages the
+    // chunk of every parent group again: at most about one new group per thread, and only
+    // while the chunks alone leave the device short of workgroups (a 256x256 lattice has
+    // 32768 chunks of 16 bytes; 13 splits of each were 22 GB of staging per step)
+    h.rs_split = std::max(1, std::min({16, ceil_div(h.cap_units, h.w == 32 ? 1024 : 256),
+                                       ceil_div(8 * device_cus(), ceil_div(h.N, h.rs_sites))}));
   } else {
     if ((size_t)h.N + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice exceeds LDS");
     h.sw = 8;
