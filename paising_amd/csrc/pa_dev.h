@@ -16,6 +16,13 @@ namespace pa {
 
 enum Coding : int32_t { kSSC = 0, kMSC = 1 };
 
+// MSC resampling of large populations by windows of new groups (k_resample_win): new groups
+// per window (one thread each), parent rows staged per 128-byte site chunk, and the
+// workgroup's dynamic LDS (the rows, a zero row, the row -> parent group table).
+constexpr int kWinJ = 256;
+constexpr int kWinSlots = 511;
+constexpr size_t kWinLds = (size_t)(kWinSlots + 1) * 128 + (size_t)kWinSlots * 4;
+
 // Mutable state of one run, advanced by the kernels.
 struct RunState {
   int32_t step;      // index i of the step in progress (1-based), == rows done + 1

@@ -100,6 +100,7 @@ struct RunHost {
   int rs_sites = 1;
   int rs_split = 1;
   int rs_lp = 1;
+  bool rs_win = false;  // MSC resampling by windows of new groups (k_resample_win)
   std::vector<double> B;
   // adaptive schedule: B grows as steps are decided; deltaBeta carries over (SSC:645-660)
   bool adaptive = false;
@@ -126,6 +127,7 @@ struct RunHost {
       // multiples of 8 blocks per run keep every run's first block on XCD 0 (the dispatcher
       // deals workgroups to the 8 XCDs round-robin); k_resample_msc relies on it
       case kKResample:
+        if (rs_win) return ceil_div(cap_units, kWinJ) * rs_split;
         return p.coding == kSSC ? ceil_div(cap_units, 8) * 8
                                 : ceil_div(ceil_div(N, rs_sites), 8 * rs_lp) * 8 * rs_lp * rs_split;
     }
@@ -133,6 +135,7 @@ struct RunHost {
   }
   size_t resample_lds() const {
     // parents' chunks plus one zero chunk
+    if (rs_win) return kWinLds;
     return p.coding == kSSC ? 0 : (size_t)(cap_units + 1) * rs_sites * (w / 8);
   }
 };
@@ -305,15 +308,26 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     h.rs_sites = 64 / (h.w / 8);
     while (h.rs_sites > 1 && (size_t)h.cap_units * h.rs_sites * (h.w / 8) > 96 * 1024) h.rs_sites /= 2;
     h.rs_lp = 128 / (h.rs_sites * (h.w / 8));
-    if ((size_t)h.cap_units * h.rs_sites * (h.w / 8) > 150 * 1024)
+    // populations whose 64-byte chunks of every parent group exceed the LDS budget are
+    // resampled by windows of new groups instead (whole 128-byte chunks: 32 / 16 words)
+    const int win_sites = 128 / (h.w / 8);
+    h.rs_win = h.rs_sites < 64 / (h.w / 8) && h.N % win_sites == 0;
+    if (h.rs_win) {
+      // windows x site-chunk ranges: about eight workgroups per CU in all
+      h.rs_sites = win_sites;
+      h.rs_split = std::max(1, std::min(h.N / win_sites,
+                                        ceil_div(8 * device_cus(), ceil_div(h.cap_units, kWinJ))));
+    } else if ((size_t)h.cap_units * h.rs_sites * (h.w / 8) > 150 * 1024) {
       return fail(PA_ERR_INVALID, "population too large for one device resampling chunk");
+    }
     // splits of a site chunk share its new groups (1024 threads per workgroup for 32-bit
     // populations above 256 groups, 256 otherwise) and run on one XCD, but each stages the
     // chunk of every parent group again: at most about one new group per thread, and only
     // while the chunks alone leave the device short of workgroups (a 256x256 lattice has
     // 32768 chunks of 16 bytes; 13 splits of each were 22 GB of staging per step)
-    h.rs_split = std::max(1, std::min({16, ceil_div(h.cap_units, h.w == 32 ? 1024 : 256),
-                                       ceil_div(8 * device_cus(), ceil_div(h.N, h.rs_sites))}));
+    if (!h.rs_win)
+      h.rs_split = std::max(1, std::min({16, ceil_div(h.cap_units, h.w == 32 ? 1024 : 256),
+                                         ceil_div(8 * device_cus(), ceil_div(h.N, h.rs_sites))}));
   } else {
     if ((size_t)h.N + 1024 > kLdsLimit) return fail(PA_ERR_INVALID, "lattice exceeds LDS");
     h.sw = 8;
@@ -498,11 +512,12 @@ struct Batch {
   int rs_nt[2] = {256, 256};  // MSC resampling threads per workgroup for w = 32, 64
   bool has_w[2] = {false, false};
   std::vector<Group> groups;
-  struct RsGroup {  // the runs of one resampling kernel (SSC, MSC32, MSC64)
+  static constexpr int kRsKinds = 5;
+  struct RsGroup {  // the runs of one resampling kernel (SSC, MSC32, MSC64, windowed 32 / 64)
     RunDesc* desc_d = nullptr;
     int32_t* offs_d = nullptr;
     int m = 0, blocks = 0;
-  } rs_groups[3];
+  } rs_groups[kRsKinds];
   bool any_trace = false;
   bool any_mhr = false;
   bool use_graph = false;
@@ -576,9 +591,9 @@ struct Batch {
     for (int r = 0; r < n; ++r) {
       descs[r] = runs[r]->desc;
       is0[r] = runs[r]->p.initial_sequence;
-      if (runs[r]->p.coding == PA_MSC) {
+      if (runs[r]->p.coding == PA_MSC) has_w[runs[r]->w == 32 ? 0 : 1] = true;
+      if (runs[r]->p.coding == PA_MSC && !runs[r]->rs_win) {
         const int wi = runs[r]->w == 32 ? 0 : 1;
-        has_w[wi] = true;
         rs_lds[wi] = std::max(rs_lds[wi], runs[r]->resample_lds());
         // 1024-thread resampling workgroups pay off for 32-bit populations above 256 groups
         if (wi == 0 && runs[r]->cap_units > 256) rs_nt[0] = 1024;
@@ -638,24 +653,24 @@ struct Batch {
         init_is0.push_back(gis_d);
       }
     }
-    // resampling tables per kernel (SSC, MSC32, MSC64): each launch covers only the runs
-    // its kernel serves, not the blocks of the other codings' runs
-    for (int k = 0; k < 3; ++k) {
+    // resampling tables per kernel (SSC, MSC32, MSC64, windowed MSC32 and MSC64): each launch
+    // covers only the runs its kernel serves, not the blocks of the other runs
+    auto rs_kind = [](const RunHost& h) {
+      return h.p.coding == PA_SSC ? 0 : (h.w == 32 ? 1 : 2) + (h.rs_win ? 2 : 0);
+    };
+    for (int k = 0; k < kRsKinds; ++k) {
       RsGroup& g = rs_groups[k];
       g = RsGroup{};
       std::vector<RunDesc> gd;
-      for (int r = 0; r < n; ++r) {
-        const RunHost& h = *runs[r];
-        const int kr = h.p.coding == PA_SSC ? 0 : h.w == 32 ? 1 : 2;
-        if (kr == k) gd.push_back(h.desc);
-      }
+      for (int r = 0; r < n; ++r)
+        if (rs_kind(*runs[r]) == k) gd.push_back(runs[r]->desc);
       g.m = (int)gd.size();
       if (!g.m) continue;
       std::vector<int32_t> go((size_t)kKindCount * (g.m + 1), 0);
       int acc = 0, i = 0;
       for (int r = 0; r < n; ++r) {
         const RunHost& h = *runs[r];
-        if ((h.p.coding == PA_SSC ? 0 : h.w == 32 ? 1 : 2) != k) continue;
+        if (rs_kind(h) != k) continue;
         go[(size_t)kKResample * (g.m + 1) + i++] = acc;
         acc += h.blocks(kKResample);
       }
@@ -741,6 +756,10 @@ struct Batch {
       if (g.m)
         HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, rs_nt[wi], g.blocks, rs_lds[wi], stream,
                                 g.desc_d, g.offs_d, g.m));
+      const RsGroup& gw = rs_groups[3 + wi];
+      if (gw.m)
+        HIPCHECK(pa_launch_step(17, PA_MSC, wi ? 64 : 32, 0, kWinJ, gw.blocks, kWinLds, stream,
+                                gw.desc_d, gw.offs_d, gw.m));
     }
     return PA_OK;
   }

@@ -839,6 +839,141 @@ __global__ __launch_bounds__(NT) void k_resample_msc(const RunDesc* __restrict__
   }
 }
 
+// MSC resampling of large populations (resampleKer MSC:397-435 again), by windows of new
+// groups.  Children n = k rg_new + j of one lane k and consecutive new groups j are
+// consecutive, and children are numbered in parent order, so for each lane the parents of a
+// window of J new groups lie in one run of parent groups: about J + 2w groups for all lanes,
+// against every parent group for k_resample_msc.  A workgroup (one thread per new group)
+// finds those runs from the window's first and last child per lane, keeps the (slot, bit) of
+// each of its 32/64 children in registers, and walks a range of whole 128-byte site chunks:
+// stage the window's parent rows of the chunk in LDS (slot T: zeros, for lanes beyond
+// R_new), gather, write one 128-byte line per new group.  A window whose parents exceed
+// kWinSlots rows is processed in halves (a single new group needs at most w rows).  Run
+// blocks: windows (capacity / J) x site-chunk ranges (rs_split).
+template <typename W>
+__global__ __launch_bounds__(256) void k_resample_win(const RunDesc* __restrict__ runs,
+                                                      const int32_t* __restrict__ offs, int nruns) {
+  constexpr int w = (int)sizeof(W) * 8;
+  constexpr int S = 128 / (int)sizeof(W);
+  constexpr int J = kWinJ;
+  const int* off = offs + kKResample * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  if (d.coding != kMSC || d.w != w) return;
+  const int Rn = st->Rnew, buf = st->buf;
+  const int rgn = ceil_div(Rn, w);
+  const int maxwin = ceil_div(d.cap_units, J);
+  const int b = blockIdx.x - off[r];
+  const int win = b % maxwin, range = b / maxwin;
+  if (win * J >= rgn) return;
+  const int N = d.N, nch = N / S;
+  const int cpr = ceil_div(nch, d.rs_split);
+  const int c0 = range * cpr, c1 = min(nch, c0 + cpr);
+  if (c0 >= c1) return;
+  extern __shared__ __align__(16) uint8_t smem_rw[];
+  uint4* old4 = (uint4*)smem_rw;
+  const W* old = (const W*)smem_rw;
+  int* slotg = (int*)(smem_rw + (size_t)(kWinSlots + 1) * 128);
+  __shared__ int kof[w + 1], kgl[w];
+  const W* src = (const W*)d.pop[buf];
+  W* dst = (W*)d.pop[buf ^ 1];
+  const int jend = min(rgn, (win + 1) * J);
+  int cur = win * J;
+  while (cur < jend) {
+    int jc = jend - cur;
+    for (;;) {
+      // rows of lane k: parent groups of its first and last child in [cur, cur + jc)
+      if (threadIdx.x < w) {
+        const int k = threadIdx.x;
+        const int64_t nlo = (int64_t)k * rgn + cur;
+        const int64_t nhi = min((int64_t)k * rgn + cur + jc, (int64_t)Rn) - 1;
+        int glo = 0, cnt = 0;
+        if (nlo <= nhi) {
+          glo = d.par[nlo] / w;
+          cnt = d.par[nhi] / w - glo + 1;
+        }
+        kgl[k] = glo;
+        kof[k + 1] = cnt;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        kof[0] = 0;
+        for (int k = 0; k < w; ++k) kof[k + 1] += kof[k];
+      }
+      __syncthreads();
+      if (kof[w] <= kWinSlots || jc == 1) break;
+      jc = (jc + 1) / 2;
+      __syncthreads();
+    }
+    const int T = kof[w];
+    for (int i = threadIdx.x; i < T; i += blockDim.x) {
+      int lo = 0, hi = w - 1;  // last lane k with kof[k] <= i
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (kof[mid] <= i) lo = mid;
+        else hi = mid - 1;
+      }
+      slotg[i] = kgl[lo] + i - kof[lo];
+    }
+    const int j = cur + (int)threadIdx.x;
+    const bool jv = (int)threadIdx.x < jc;
+    uint32_t dsc[w / 2];  // (slot << 6 | bit) of lanes 2i and 2i + 1 in the halves of dsc[i]
+#pragma unroll
+    for (int k = 0; k < w; ++k) {
+      const int64_t n = (int64_t)k * rgn + j;
+      uint32_t v = (uint32_t)T << 6;
+      if (jv && n < Rn) {
+        const int p = d.par[n];
+        v = ((uint32_t)(kof[k] + p / w - kgl[k]) << 6) | (uint32_t)(p % w);
+      }
+      if (k & 1) dsc[k >> 1] |= v << 16;
+      else dsc[k >> 1] = v;
+    }
+    if (jv && c0 == 0) {
+      uint64_t m = 0;
+      for (int k = 0; k < w; ++k)
+        if ((int64_t)k * rgn + j < Rn) m |= 1ull << k;
+      d.amask[buf ^ 1][j] = m;
+    }
+    __syncthreads();
+    for (int c = c0; c < c1; ++c) {
+      const int s0 = c * S;
+      for (int idx = threadIdx.x; idx < T * 8; idx += blockDim.x) {
+        const int i = idx >> 3, q = idx & 7;
+        old4[idx] = ((const uint4*)(src + (size_t)slotg[i] * N + s0))[q];
+      }
+      if (threadIdx.x < 8) old4[T * 8 + threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+      __syncthreads();
+      if (jv) {
+        W acc[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[s] = 0;
+#pragma unroll
+        for (int k = 0; k < w; ++k) {
+          const uint32_t v = (dsc[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
+          const W* o = old + (v >> 6) * S;
+          const unsigned pb = v & 63u;
+#pragma unroll
+          for (int s = 0; s < S; ++s) acc[s] |= ((o[s] >> pb) & (W)1) << k;
+        }
+        uint4* out = (uint4*)(dst + (size_t)j * N + s0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if constexpr (sizeof(W) == 4)
+            out[q] = make_uint4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
+          else
+            out[q] = make_uint4((uint32_t)acc[2 * q], (uint32_t)(acc[2 * q] >> 32),
+                                (uint32_t)acc[2 * q + 1], (uint32_t)(acc[2 * q + 1] >> 32));
+        }
+      }
+      __syncthreads();
+    }
+    cur += jc;
+  }
+}
+
 // ============================================================================================
 // Sweeps.  The unit's lattice lives in LDS for all EqSw sweeps of the step.  Site (side h,
 // idx) of sweep s is driven by stream t = idx mod 128 of the unit (subsequence
@@ -1884,6 +2019,8 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW((k_resample_msc<uint32_t, 256>));
   PA_ALLOW((k_resample_msc<uint32_t, 1024>));
   PA_ALLOW((k_resample_msc<uint64_t, 256>));
+  PA_ALLOW((k_resample_win<uint32_t>));
+  PA_ALLOW((k_resample_win<uint64_t>));
   PA_ALLOW((k_sweep_ssc<128, 0>));
   PA_ALLOW((k_sweep_ssc<256, 0>));
   PA_ALLOW((k_sweep_ssc<512, 0>));
@@ -2035,6 +2172,12 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
         hipLaunchKernelGGL((k_resample_msc<uint32_t, 256>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
       else
         hipLaunchKernelGGL((k_resample_msc<uint64_t, 256>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+      break;
+    case 17:  // MSC resampling by windows of new groups (whole 128-byte site chunks)
+      if (w == 32)
+        hipLaunchKernelGGL((k_resample_win<uint32_t>), dim3(nblocks), dim3(kWinJ), kWinLds, s, runs, offs, nruns);
+      else
+        hipLaunchKernelGGL((k_resample_win<uint64_t>), dim3(nblocks), dim3(kWinJ), kWinLds, s, runs, offs, nruns);
       break;
     default: return hipErrorInvalidValue;
   }

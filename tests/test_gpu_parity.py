@@ -19,6 +19,9 @@ CASES = [
     (24, pa.MSC, 32, 300, 3, 0.02, 1.0, 11, 5),
     (32, pa.MSC, 64, 1500, 6, 0.005, 1.0, 3, 5),
     (16, pa.MSC, 64, 200, 4, 0.05, 1.0, 9, 6),
+    # populations whose parent chunks exceed the LDS budget: resampling by windows
+    (16, pa.MSC, 32, 60000, 2, 0.05, 1.0, 13, 3),
+    (16, pa.MSC, 64, 110000, 2, 0.05, 1.0, 17, 3),
 ]
 
 

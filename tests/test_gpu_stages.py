@@ -163,6 +163,42 @@ def test_resample_stage(coding, L, w):
         assert Rn == Rn_ref and (got == ref).all() and (an == ran).all()
 
 
+@pytest.mark.parametrize("L,w,R,pattern", [(16, 32, 60000, "uniform"), (16, 32, 60000, "shrink"),
+                                           (24, 64, 100003, "uniform"), (16, 64, 100000, "shrink"),
+                                           (16, 64, 100000, "few"), (20, 32, 60000, "uniform")])
+def test_resample_stage_large_populations(L, w, R, pattern):
+    """Populations whose parent chunks exceed the LDS budget (resampling by windows of new
+    groups where N is a multiple of the 128-byte chunk, L=20 with 16-byte chunks otherwise):
+    copy counts of 0-2, of 0-1 with the population shrinking fourfold (windows processed in
+    halves), and of 0 or 50 (few parents, long runs of children); inactive lanes inside."""
+    rng = np.random.default_rng(R + w + L)
+    p = params(L, pa.MSC, w, Rinit=R)
+    rg = math.ceil(R / w)
+    n = rg * w
+    pop = msc_pop(L, w, R, rng)
+    active = (np.arange(n) < R).astype(np.uint8)
+    active[rng.integers(0, R, 200)] = 0
+    if pattern == "uniform":
+        Roff = rng.integers(0, 3, n)
+    elif pattern == "shrink":
+        Roff = (rng.random(n) < 0.25).astype(np.int64)
+    else:
+        Roff = np.where(rng.random(n) < 0.02, 50, 0)
+    Roff = (Roff * active).astype(np.uint32)
+    Rn_ref = int(Roff.sum())
+    print(f"case MSC resample L={L} w={w} R={R} {pattern} Rn={Rn_ref}", flush=True)
+    got, an, Rn = pa.stage_resample(p, pop, R, Roff, active)
+    rgn = math.ceil(Rn_ref / w)
+    dt = np.uint32 if w == 32 else np.uint64
+    ref = np.zeros((rgn, L * L), dt)
+    ran = np.zeros(rgn * w, np.uint8)
+    IE = np.zeros(n, np.int32)
+    IEn = np.zeros(rgn * w, np.int32)
+    lib.ora_msc_resample(L, w, rg, P(np.ascontiguousarray(pop)), P(active), P(IE), P(Roff),
+                         Rn_ref, P(ref), P(ran), P(IEn))
+    assert Rn == Rn_ref and (got == ref).all() and (an == ran).all()
+
+
 @pytest.mark.parametrize("coding,w,R", [(pa.SSC, 32, 5000), (pa.MSC, 32, 4001), (pa.MSC, 64, 3000)])
 def test_averages_stage(coding, w, R):
     rng = np.random.default_rng(R)
