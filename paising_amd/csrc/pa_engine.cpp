@@ -265,14 +265,18 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   if (h.nt != 128 && h.nt != 256 && h.nt != 512 && !(h.nt == 1024 && p.coding == PA_MSC))
     return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512 (or 1024 for MSC)");
   if (!aligned && p.coding == PA_SSC && ceil_div(Nh, 128) <= 9) h.nt = 128;
-  // SSC sweep kernel path (pa_kernels.hip, ssc_sweeps): register-cached block loop for two
-  // or three Philox blocks per thread and half sweep (1), fixed-column block loop for more
-  // blocks when L/2 divides 128 (4), register-cached stream walk for unaligned lattices with
-  // at most 4 / 9 sites per stream and half sweep (2 / 3), blocks dealt to several threads
-  // per stream beyond that (5)
+  // SSC sweep kernel path (pa_kernels.hip, ssc_sweeps): columns repeating every third site
+  // of a stream (L = 96, 192) for more than one Philox block per thread and half sweep (6),
+  // register-cached block loop for two or three blocks per thread (1), fixed-column block
+  // loop for more blocks when L/2 divides 128 (4), register-cached stream walk for unaligned
+  // lattices with at most 4 / 9 sites per stream and half sweep (2 / 3), blocks dealt to
+  // several threads per stream beyond that (5)
   if (p.coding == PA_SSC) {
     const int ntmax = ceil_div(Nh, 128);
-    if (aligned && nb > h.nt / 128 && nb <= 3 * (h.nt / 128)) h.path = 1;
+    const int Ld2 = p.L / 2;
+    const bool period3 = 128 % Ld2 != 0 && 384 % Ld2 == 0 && (384 / Ld2) % 2 == 0;
+    if (aligned && period3 && nb > h.nt / 128) h.path = 6;
+    else if (aligned && nb > h.nt / 128 && nb <= 3 * (h.nt / 128)) h.path = 1;
     else if (aligned && nb > 3 * (h.nt / 128) && 128 % (p.L / 2) == 0) h.path = 4;
     else if (!aligned && ntmax <= 4) h.path = 2;
     else if (!aligned && ntmax <= 9) h.path = 3;

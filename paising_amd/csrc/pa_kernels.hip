@@ -1032,7 +1032,10 @@ __device__ __forceinline__ void ssc_walk_cached(int8_t* lat8, const RunDesc& d, 
 //   0: N/2 a multiple of 512 with at most NT/128 Philox blocks per thread and half sweep
 //      (fast), or the uncached block loop / stream walk for anything larger;
 //   1: aligned, two or three blocks per thread (register-cached block loop);
-//   2, 3: not aligned, at most 4 / 9 sites per stream (register-cached stream walk).
+//   2, 3: not aligned, at most 4 / 9 sites per stream (register-cached stream walk);
+//   4: aligned, more blocks per thread, L/2 dividing 128 (each thread on one column);
+//   5: not aligned, more than 9 sites per stream (Philox blocks dealt to P threads);
+//   6: aligned, columns of a stream's sites repeating every third site (L = 96, 192).
 template <int NT, bool ALWAYS1, int PATH>
 __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint64_t seed,
                                            uint64_t sub0, uint32_t nsw,
@@ -1079,6 +1082,66 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
             sp[jj] = own[o];
           }
           ssc_flip4<ALWAYS1>(own, oth, q, sp, rb, tab);
+        }
+        __syncthreads();
+      }
+    }
+    return;
+  }
+  if constexpr (PATH == 6) {
+    // N/2 a multiple of 512 and 3 x 128 a multiple of L with an even quotient, Ld2 not
+    // dividing 128 (L = 96, 192): site t + 128 k of stream t lies in column
+    // (t + 128 k) mod Ld2, which repeats every third k while the row advances by an even
+    // number, so the horizontal neighbour's offset from the site depends only on
+    // (k mod 3, half).  Vertical neighbours are the site -/+ Ld2, except above the stream's
+    // first site (t < Ld2) and below its last (t >= 128 - Ld2).  The offsets of a block's
+    // four sites are its base plus constants; six registers hold the horizontal offsets.
+    const int Ld2 = d.Ld2, nb = Nh / 512;
+    int ec[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Site q = site_at(t + kStreamsPerUnit * c, h, d);
+        ec[h][c] = q.i1 - q.idx;
+      }
+    // k = 4 (part + P j) + jj, so k mod 3 = (part + P j + jj) mod 3: rotate by part
+    const int rot = part % 3;
+    int er[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int y = 0; y < 3; ++y)
+        er[h][y] = rot == 0 ? ec[h][y] : rot == 1 ? ec[h][(y + 1) % 3] : ec[h][(y + 2) % 3];
+    const int top = t < Ld2 ? Nh : 0;
+    const int bot = t >= kStreamsPerUnit - Ld2 ? Nh : 0;
+    for (uint32_t s = 0; s < nsw; ++s) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        int8_t* own = lat8 + h * Nh;
+        const int8_t* oth = lat8 + (h ^ 1) * Nh;
+        for (int j0 = 0; part + P * j0 < nb; j0 += 3) {
+#pragma unroll
+          for (int jr = 0; jr < 3; ++jr) {
+            const int m = part + P * (j0 + jr);
+            if (m < nb) {
+              const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+              const int o0 = t + 512 * m;
+              const int up = m == 0 ? top : 0, dn = m == nb - 1 ? bot : 0;
+              Site q[4];
+              int sp[4];
+#pragma unroll
+              for (int jj = 0; jj < 4; ++jj) {
+                const int o = o0 + jj * kStreamsPerUnit;
+                q[jj].idx = o;
+                q[jj].i1 = o + er[h][(P * jr + jj) % 3];
+                q[jj].i3 = o - Ld2 + (jj == 0 ? up : 0);
+                q[jj].i4 = o + Ld2 - (jj == 3 ? dn : 0);
+                sp[jj] = own[o];
+              }
+              ssc_flip4<ALWAYS1>(own, oth, q, sp, rb, tab);
+            }
+          }
         }
         __syncthreads();
       }
@@ -2033,6 +2096,9 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW((k_sweep_ssc<256, 4>));
   PA_ALLOW((k_sweep_ssc<512, 4>));
   PA_ALLOW((k_sweep_ssc<128, 5>));
+  PA_ALLOW((k_sweep_ssc<128, 6>));
+  PA_ALLOW((k_sweep_ssc<256, 6>));
+  PA_ALLOW((k_sweep_ssc<512, 6>));
   PA_ALLOW((k_sweep_ssc<256, 5>));
   PA_ALLOW((k_sweep_ssc<512, 5>));
 #define PA_ALLOW_MSC(WT, ST)                          \
@@ -2138,6 +2204,10 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
             else if (nt == 128) hipLaunchKernelGGL((k_sweep_ssc<128, 5>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
             else return hipErrorInvalidValue;
           }
+        } else if (sw == 6) {
+          if (nt == 512) hipLaunchKernelGGL((k_sweep_ssc<512, 6>), dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
+          else if (nt == 256) hipLaunchKernelGGL((k_sweep_ssc<256, 6>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+          else hipLaunchKernelGGL((k_sweep_ssc<128, 6>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
         } else if (sw == 1) {
           if (nt == 512) hipLaunchKernelGGL((k_sweep_ssc<512, 1>), dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
           else if (nt == 256) hipLaunchKernelGGL((k_sweep_ssc<256, 1>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);

@@ -97,7 +97,7 @@ def test_ssc_block_shapes_match_oracle(nt):
     assert compare_rows(orows, erows) == 4
 
 
-@pytest.mark.parametrize("L,nt", [(96, 0), (96, 256), (48, 0), (16, 0), (10, 0), (6, 0), (4, 0),
+@pytest.mark.parametrize("L,nt", [(96, 0), (96, 256), (96, 128), (192, 0), (192, 256), (48, 0), (16, 0), (10, 0), (6, 0), (4, 0),
                                   (130, 0), (160, 0), (64, 128), (128, 0), (128, 128), (256, 0),
                                   (80, 0), (144, 256), (112, 128)])
 def test_ssc_lattices_off_the_fast_path_match_oracle(L, nt):

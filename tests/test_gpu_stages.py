@@ -33,7 +33,8 @@ def msc_pop(L, w, R, rng):
 
 
 @pytest.mark.parametrize("L,beta,uniform,nt", [(16, 0.44, 0, 0), (24, 1.0, 1, 0), (64, 0.0, 0, 0),
-                                               (64, 0.3, 1, 256), (96, 0.7, 0, 0), (48, 0.5, 0, 0),
+                                               (64, 0.3, 1, 256), (96, 0.7, 0, 0), (96, 0.44, 1, 128),
+                                               (192, 0.44, 0, 0), (48, 0.5, 0, 0),
                                                (144, 0.44, 0, 0)])
 def test_ssc_sweep_stage(L, beta, uniform, nt):
     rng = np.random.default_rng(L + nt)
