@@ -1097,6 +1097,9 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
     // first site (t < Ld2) and below its last (t >= 128 - Ld2).  The offsets of a block's
     // four sites are its base plus constants; six registers hold the horizontal offsets.
     const int Ld2 = d.Ld2, nb = Nh / 512;
+    // part is uniform per wave (128 threads per part): as a scalar, the block index, the
+    // Philox counter and the wrap tests below stay on the scalar unit
+    const int partu = __builtin_amdgcn_readfirstlane(part);
     int ec[2][3];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
@@ -1106,7 +1109,7 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
         ec[h][c] = q.i1 - q.idx;
       }
     // k = 4 (part + P j) + jj, so k mod 3 = (part + P j + jj) mod 3: rotate by part
-    const int rot = part % 3;
+    const int rot = partu % 3;
     int er[2][3];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
@@ -1120,10 +1123,10 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
       for (int h = 0; h < 2; ++h) {
         int8_t* own = lat8 + h * Nh;
         const int8_t* oth = lat8 + (h ^ 1) * Nh;
-        for (int j0 = 0; part + P * j0 < nb; j0 += 3) {
+        for (int j0 = 0; partu + P * j0 < nb; j0 += 3) {
 #pragma unroll
           for (int jr = 0; jr < 3; ++jr) {
-            const int m = part + P * (j0 + jr);
+            const int m = partu + P * (j0 + jr);
             if (m < nb) {
               const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
               const int o0 = t + 512 * m;
