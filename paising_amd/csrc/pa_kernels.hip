@@ -1048,7 +1048,12 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
     return;
   }
   constexpr int P = NT / kStreamsPerUnit;
-  const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
+  // part is uniform per wave (a stream part is two waves): as a scalar, block indices,
+  // Philox counters and per-block tests stay on the scalar unit (not in the register-cached
+  // block loop, whose offsets are set up per thread and which then needs more registers)
+  const int t = threadIdx.x % kStreamsPerUnit;
+  const int part = PATH == 1 ? (int)(threadIdx.x / kStreamsPerUnit)
+                             : __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kStreamsPerUnit));
   if constexpr (PATH == 4) {
     // N/2 a multiple of 512 with more than 3 blocks per thread and L/2 dividing 128
     // (L = 128, 256): every site of thread t lies in column t % Ld2 and site t + 128 k in row
@@ -1097,9 +1102,6 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
     // first site (t < Ld2) and below its last (t >= 128 - Ld2).  The offsets of a block's
     // four sites are its base plus constants; six registers hold the horizontal offsets.
     const int Ld2 = d.Ld2, nb = Nh / 512;
-    // part is uniform per wave (128 threads per part): as a scalar, the block index, the
-    // Philox counter and the wrap tests below stay on the scalar unit
-    const int partu = __builtin_amdgcn_readfirstlane(part);
     int ec[2][3];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
@@ -1109,7 +1111,7 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
         ec[h][c] = q.i1 - q.idx;
       }
     // k = 4 (part + P j) + jj, so k mod 3 = (part + P j + jj) mod 3: rotate by part
-    const int rot = partu % 3;
+    const int rot = part % 3;
     int er[2][3];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
@@ -1123,10 +1125,10 @@ __device__ __forceinline__ void ssc_sweeps(int8_t* lat8, const RunDesc& d, uint6
       for (int h = 0; h < 2; ++h) {
         int8_t* own = lat8 + h * Nh;
         const int8_t* oth = lat8 + (h ^ 1) * Nh;
-        for (int j0 = 0; partu + P * j0 < nb; j0 += 3) {
+        for (int j0 = 0; part + P * j0 < nb; j0 += 3) {
 #pragma unroll
           for (int jr = 0; jr < 3; ++jr) {
-            const int m = partu + P * (j0 + jr);
+            const int m = part + P * (j0 + jr);
             if (m < nb) {
               const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
               const int o0 = t + 512 * m;
@@ -1380,7 +1382,8 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
   __syncthreads();
   if (FAST || (Nh & 511) == 0) {
     constexpr int P = NT / kStreamsPerUnit;
-    const int t = threadIdx.x % kStreamsPerUnit, part = threadIdx.x / kStreamsPerUnit;
+    const int t = threadIdx.x % kStreamsPerUnit;
+    const int part = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kStreamsPerUnit));
     const int nb = Nh / 512;
     auto block4 = [&](S* own, const S* oth, const Site (&q)[4], const U4& rb) {
       uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
