@@ -50,7 +50,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload (FETCH_SIZE scaled x2, WRITE_SIZE x1, the factors a plain device copy of known
 # size reads back): profiles/k_sweep_msc_bal_msc32_L64_R20000.md, k_resample_L64_R20000.md.
-TRAFFIC_BYTES = {"sweep_msc32": 41.73e6, "resample_msc32": 21.41e6}
+TRAFFIC_BYTES = {"sweep_msc32": 37.63e6, "resample_msc32": 21.41e6}
 
 
 def msc_ops_per_attempt(w=32):
