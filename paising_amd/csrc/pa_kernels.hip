@@ -1479,6 +1479,63 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
           __syncthreads();
         }
       }
+    } else if (384 % Ld2 == 0 && ((384 / Ld2) & 1) == 0) {
+      // 3 x 128 an even multiple of Ld2 (L = 96, 192): site t + 128 k of stream t lies in
+      // column (t + 128 k) mod Ld2, which repeats every third k while the row advances by an
+      // even number, so the horizontal neighbour's offset depends only on (k mod 3, half);
+      // k = 4 (part + P j) + jj gives k mod 3 = (part + P j + jj) mod 3.  Vertical
+      // neighbours wrap only above the stream's first site and below its last.
+      constexpr int B = (int)sizeof(S);
+      int ec[2][3];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const Site q = site_at(t + kStreamsPerUnit * c, h, d);
+          ec[h][c] = (q.i1 - q.idx) * B;
+        }
+      const int rot = part % 3;
+      int er[2][3];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+          er[h][y] = rot == 0 ? ec[h][y] : rot == 1 ? ec[h][(y + 1) % 3] : ec[h][(y + 2) % 3];
+      const int dV = Ld2 * B, dN = Nh * B;
+      const int top = t < Ld2 ? dN : 0, bot = t >= kStreamsPerUnit - Ld2 ? dN : 0;
+      for (uint32_t s = s_begin; s < s_end; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          uint8_t* own = (uint8_t*)(lat + h * Nh);
+          const uint8_t* oth = (const uint8_t*)(lat + (h ^ 1) * Nh);
+          for (int j0 = 0; part + P * j0 < nb; j0 += 3) {
+#pragma unroll
+            for (int jr = 0; jr < 3; ++jr) {
+              const int m = part + P * (j0 + jr);
+              if (m < nb) {
+                const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+                uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
+                S c4[4], c8[4];
+                lcg_masks<S, 4>(ran, T4, T8, c4, c8);
+                const int o0 = (t + 512 * m) * B;
+                const int up = m == 0 ? top : 0, dn = m == nb - 1 ? bot : 0;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                  const int o = o0 + jj * 128 * B;
+                  const int o1 = o + er[h][(P * jr + jj) % 3];
+                  const int o3 = o - dV + (jj == 0 ? up : 0);
+                  const int o4 = o + dV - (jj == 3 ? dn : 0);
+                  const S sp = *(const S*)(own + o);
+                  *(S*)(own + o) = sp ^ msc_acc<S>(sp, *(const S*)(oth + o1), *(const S*)(oth + o),
+                                                   *(const S*)(oth + o3), *(const S*)(oth + o4),
+                                                   c4[jj], c8[jj]);
+                }
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
     } else {
       for (uint32_t s = s_begin; s < s_end; ++s) {
         for (int h = 0; h < 2; ++h) {
