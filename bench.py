@@ -117,14 +117,16 @@ def cpu_baseline(sample_steps, threads):
 
 def timed_run(coding, steps, warmup, tune=True):
     """Plain path at the reference sample shape: warmup steps, then `steps` steps timed on
-    the host around the launches + one sync (the throughput), then `steps` more steps with
-    HIP events on the run's stream around each step's resampling and sweep launches (the
-    kernel times; the events cost a few microseconds of idle device per step, so they stay
-    out of the throughput pass).  Returns attempts, wall s, kernel times, tuned variant and
-    rows."""
+    the host around the launches + one sync (the throughput), then up to `steps` more steps
+    (as many as the schedule has left) with HIP events on the run's stream around each
+    step's resampling and sweep launches (the kernel times; the events cost a few
+    microseconds of idle device per step, so they stay out of the throughput pass; with no
+    schedule left the throughput steps carry the events).  Returns attempts, wall s, kernel
+    times, tuned variant and rows."""
     total = pa.schedule_steps(0.0, DB, 1.0)
-    if warmup + 2 * steps > total:
-        raise SystemExit(f"--warmup + 2 x --steps must be <= {total}")
+    if warmup + steps > total:
+        raise SystemExit(f"--warmup + --steps must be <= {total}")
+    ev_steps = min(steps, total - warmup - steps)
     p = pa.make_params(L=L, coding=coding, msbits=32, Rinit=RINIT, eqsweeps=EQSW, dB=DB, Bfin=1.0,
                        seed=SEED)
     if tune:
@@ -132,25 +134,29 @@ def timed_run(coding, steps, warmup, tune=True):
     run = pa.Run(p)
     run.step(warmup)
     run.sync()
+    if ev_steps < 1:
+        run.set_timing(True)
     t0 = time.perf_counter()
     run.step(steps)
     run.sync()
     wall = time.perf_counter() - t0
-    run.set_timing(True)
-    run.step(steps)
-    run.sync()
+    if ev_steps >= 1:
+        run.set_timing(True)
+        run.step(ev_steps)
+        run.sync()
     rs_ms, sw_ms, nsteps = run.kernel_times()
     rows = run.results()
     run.close()
     timed = rows[warmup + 1: warmup + 1 + steps]
-    evrows = rows[warmup + 1 + steps: warmup + 1 + 2 * steps]
-    assert nsteps == steps and len(timed) == steps and len(evrows) == steps
+    evrows = rows[warmup + 1 + steps: warmup + 1 + steps + ev_steps] if ev_steps >= 1 else timed
+    assert len(timed) == steps and nsteps == len(evrows)
     attempts = EQSW * N * sum(r.R for r in timed)
     ev_attempts = EQSW * N * sum(r.R for r in evrows)
     # resampling reads the parents and writes the children: N bits (MSC) or bytes (SSC) each
     bpr = N / 8 if coding == pa.MSC else N
     rs_bytes = sum(bpr * (r.Rprev + r.R) for r in evrows)
-    return dict(attempts=attempts, wall=wall, ev_attempts=ev_attempts, sweep_ms=sw_ms,
+    return dict(attempts=attempts, wall=wall, ev_attempts=ev_attempts, ev_steps=len(evrows),
+                sweep_ms=sw_ms,
                 resample_ms=rs_ms, rs_bytes=rs_bytes,
                 variant={"slice_bits": p.slice_bits, "threads": p.threads, "balance": p.balance},
                 rows=rows)
@@ -407,13 +413,13 @@ def main():
                      "unit": "Tlane-op/s", "frac": round(achieved / peak, 4),
                      "traffic": TRAFFIC_BYTES["sweep_msc32"],
                      "ops_per_attempt": round(ops, 4),
-                     "sweep_ms_per_step": round(m["sweep_ms"] / args.steps, 4),
+                     "sweep_ms_per_step": round(m["sweep_ms"] / m["ev_steps"], 4),
                      "sweep_attempts_per_ns": round(m["ev_attempts"] / (m["sweep_ms"] * 1e6), 2)},
         "roofline_resample": {"bound": "hbm", "kernel": "k_resample_msc",
                               "achieved": round(rs_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(rs_gbs / HBM_PEAK_GBS, 4),
                               "traffic": TRAFFIC_BYTES["resample_msc32"],
-                              "resample_ms_per_step": round(m["resample_ms"] / args.steps, 4)},
+                              "resample_ms_per_step": round(m["resample_ms"] / m["ev_steps"], 4)},
     }
     if not args.no_ssc:
         s_ = timed_run(pa.SSC, args.steps, args.warmup, tune=not args.no_tune)
