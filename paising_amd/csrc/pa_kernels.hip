@@ -1479,12 +1479,13 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
           __syncthreads();
         }
       }
-    } else if (384 % Ld2 == 0 && ((384 / Ld2) & 1) == 0) {
-      // 3 x 128 an even multiple of Ld2 (L = 96, 192): site t + 128 k of stream t lies in
+    } else if (Ld2 < 128 && 384 % Ld2 == 0 && ((384 / Ld2) & 1) == 0) {
+      // 3 x 128 an even multiple of Ld2 below 128 (L = 96, 192): site t + 128 k of stream t lies in
       // column (t + 128 k) mod Ld2, which repeats every third k while the row advances by an
       // even number, so the horizontal neighbour's offset depends only on (k mod 3, half);
       // k = 4 (part + P j) + jj gives k mod 3 = (part + P j + jj) mod 3.  Vertical
-      // neighbours wrap only above the stream's first site and below its last.
+      // neighbours wrap only above the stream's first site and below its last, which holds
+      // only while Ld2 < 128 (at L = 384 the first two sites of a stream share row 0).
       constexpr int B = (int)sizeof(S);
       int ec[2][3];
 #pragma unroll

@@ -76,12 +76,15 @@ def test_balanced_batch_of_mixed_sizes():
                                        (10, 32, 32, 256), (16, 64, 64, 512), (112, 32, 0, 0),
                                        (176, 64, 0, 0), (24, 32, 0, 0), (40, 32, 0, 0),
                                        (192, 32, 32, 1024), (192, 32, 0, 0), (160, 32, 0, 0),
-                                       (256, 32, 32, 0), (256, 64, 32, 0), (256, 64, 32, 1024)])
+                                       (256, 32, 32, 0), (256, 64, 32, 0), (256, 64, 32, 1024),
+                                       (384, 32, 8, 0), (384, 64, 8, 512)])
 def test_slices_and_block_shapes_match_oracle(L, w, sw, nt):
     """Every replica-bit slice width and block shape the autotuner may pick is bit-exact,
     on lattices whose half sweeps split into whole Philox blocks per stream and on lattices
     whose do not (L = 10, 16, 24, 40, 48, 80, 112, 144, 176); sw = nt = 0 is the default plan.
-    L = 256 with 32-bit slices keeps one sublattice in LDS and the other in registers."""
+    L = 256 with 32-bit slices keeps one sublattice in LDS and the other in registers.
+    L = 384 (L/2 = 192 > 128: two sites of a stream in row 0) must not take the
+    periodic-column path of L = 96 and 192."""
     args = (L, pa.MSC, w, 700, 4, 0.01, 1.0, 13, 4)
     print(f"case L={L} w={w} sw={sw} nt={nt}", flush=True)
     orows = oracle_rows(*args)
@@ -99,7 +102,7 @@ def test_ssc_block_shapes_match_oracle(nt):
 
 @pytest.mark.parametrize("L,nt", [(96, 0), (96, 256), (96, 128), (192, 0), (192, 256), (48, 0), (16, 0), (10, 0), (6, 0), (4, 0),
                                   (130, 0), (160, 0), (64, 128), (128, 0), (128, 128), (256, 0),
-                                  (80, 0), (144, 256), (112, 128)])
+                                  (80, 0), (144, 256), (112, 128), (384, 0), (384, 128)])
 def test_ssc_lattices_off_the_fast_path_match_oracle(L, nt):
     """SSC lattices whose half sweeps do not split into whole Philox blocks per thread
     (stream walk), or need several blocks per thread, at every block shape."""
