@@ -198,10 +198,15 @@ int pa_batch_destroy(pa_batch* b);
 /* ---- serving (no reference counterpart): continuous batching under an HBM budget ---- */
 int pa_server_create(size_t hbm_budget_bytes, int max_runs, pa_server** out);
 int pa_server_submit(pa_server* s, const pa_params* p, int64_t* ticket);
-/* Advance the device by one batched step of every resident run, admitting queued requests
- * as capacity frees up; returns the number of requests completed so far in *completed. */
+/* One tick: pick up the results of runs whose last step has completed, admit queued
+ * requests while their footprint fits the budget (a request whose device state cannot be
+ * created fails alone), and enqueue the next steps of the scheduled resident runs (see
+ * pa_server_set_chunk / pa_server_set_policy).  Asynchronous: at most two ticks are in
+ * flight, and a tick never waits for the device to drain.  *completed = requests finished
+ * (or failed) so far. */
 int pa_server_tick(pa_server* s, int* completed);
-/* 1 when the ticket is finished, 0 when queued or running. */
+/* *done = 1 when the ticket has finished or failed, 0 when queued or running; a failed
+ * request returns its error code. */
 int pa_server_poll(pa_server* s, int64_t ticket, int* done);
 int pa_server_results(pa_server* s, int64_t ticket, pa_row* rows, int max_rows, int* nrows);
 int pa_server_stats(pa_server* s, int* queued, int* resident, size_t* hbm_used);
@@ -214,6 +219,14 @@ int pa_server_set_chunk(pa_server* s, int steps_per_tick, int use_graph);
  * step), in that order, until their workgroups reach four per CU; the rest pause for the tick.
  * Results do not depend on the policy.  No counterpart in the reference (serving only). */
 int pa_server_set_policy(pa_server* s, int policy);
+/* Device memory of the server (no reference counterpart).  Run arenas come from slabs of
+ * device memory that only grow (at least 4 GiB each, within the budget) and are reused after
+ * runs retire, so admissions and retirements neither allocate nor free device memory once
+ * the pool holds the resident load.  pa_server_reserve grows the arena pool to `bytes` (at
+ * most the budget) up front; pa_server_memory reports the device bytes the server holds
+ * (arena and table slabs) and how many device allocations it has made so far. */
+int pa_server_reserve(pa_server* s, size_t bytes);
+int pa_server_memory(pa_server* s, size_t* reserved, int* device_allocations);
 /* Seconds from submission (and from admission) to completion of a finished request. */
 int pa_server_latency(pa_server* s, int64_t ticket, double* submit_to_done_s,
                       double* admit_to_done_s);

@@ -98,6 +98,8 @@ SIGNATURES = {
     "pa_stage_resample": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "pa_stage_averages": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "pa_server_latency": (_I, [_P, ctypes.c_int64, _P, _P]),
+    "pa_server_reserve": (_I, [_P, _SZ]),
+    "pa_server_memory": (_I, [_P, _P, _P]),
     "pa_server_destroy": (_I, [_P]),
     "pa_run_footprint": (_SZ, [_P]),
     "pa_autotune": (_I, [_P, _P, _P]),
@@ -474,6 +476,17 @@ class Server:
         b = ctypes.c_double(0)
         _check(lib().pa_server_latency(self.h, ticket, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def reserve(self, nbytes):
+        """Grow the device pool of run arenas to `nbytes` (at most the budget) up front."""
+        _check(lib().pa_server_reserve(self.h, nbytes))
+
+    def memory(self):
+        """(device bytes held by the server's pools, device allocations made so far)."""
+        r = ctypes.c_size_t(0)
+        n = ctypes.c_int(0)
+        _check(lib().pa_server_memory(self.h, ctypes.byref(r), ctypes.byref(n)))
+        return r.value, n.value
 
     def stats(self):
         q = ctypes.c_int(0)

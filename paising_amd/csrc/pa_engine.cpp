@@ -91,6 +91,272 @@ struct SweepVariant {
   }
 };
 
+// ---- memory of the server: grow-only pools, stream-ordered copies, deferred frees ----
+// A plain run or batch allocates its device memory once, at creation, with hipMalloc.  The
+// server admits and retires runs while earlier ticks are still on the device, so it must not
+// allocate or free device memory per request (hipFree waits for the whole device).  It
+// carves run arenas and batch tables out of large device slabs that only grow, stages host
+// data in pinned slabs so copies stay asynchronous on the server stream, and returns a block
+// to its pool only after an event recorded behind the last work that may touch it has
+// completed.
+
+// First-fit allocator over slabs of one kind of memory (device or pinned host); blocks are
+// 256-byte aligned, free neighbours coalesce.
+class SlabPool {
+ public:
+  explicit SlabPool(bool host) : host_(host) {}
+  ~SlabPool() { release_all(); }
+  SlabPool(const SlabPool&) = delete;
+  SlabPool& operator=(const SlabPool&) = delete;
+
+  // A free block of `bytes`, or null when no slab has room (the caller may grow()).
+  void* take(size_t bytes) {
+    bytes = round(bytes);
+    for (Slab& s : slabs_) {
+      for (auto it = s.free_.begin(); it != s.free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, len = it->second;
+        s.free_.erase(it);
+        if (len > bytes) s.free_[off + bytes] = len - bytes;
+        uint8_t* p = s.base + off;
+        used_[p] = bytes;
+        in_use_ += bytes;
+        return p;
+      }
+    }
+    return nullptr;
+  }
+  // Returns a block to its slab (the caller guarantees that no queued work touches it).
+  void give(void* ptr) {
+    auto u = used_.find((uint8_t*)ptr);
+    if (u == used_.end()) return;
+    const size_t bytes = u->second;
+    used_.erase(u);
+    in_use_ -= bytes;
+    for (Slab& s : slabs_) {
+      uint8_t* p = (uint8_t*)ptr;
+      if (p < s.base || p >= s.base + s.size) continue;
+      size_t off = (size_t)(p - s.base), len = bytes;
+      auto next = s.free_.lower_bound(off);
+      if (next != s.free_.end() && off + len == next->first) {
+        len += next->second;
+        next = s.free_.erase(next);
+      }
+      if (next != s.free_.begin()) {
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+          prev->second += len;
+          return;
+        }
+      }
+      s.free_[off] = len;
+      return;
+    }
+  }
+  // One more slab of at least `bytes`.
+  hipError_t grow(size_t bytes) {
+    bytes = round(bytes);
+    void* p = nullptr;
+    hipError_t e = host_ ? hipHostMalloc(&p, bytes, hipHostMallocDefault) : hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();  // the failure is reported to the caller, not left pending
+      return e;
+    }
+    Slab s;
+    s.base = (uint8_t*)p;
+    s.size = bytes;
+    s.free_[0] = bytes;
+    slabs_.push_back(std::move(s));
+    reserved_ += bytes;
+    ++grows_;
+    return hipSuccess;
+  }
+  // Frees the slabs without a block in use (only when the device is idle: hipFree waits).
+  void trim() {
+    for (size_t i = 0; i < slabs_.size();) {
+      Slab& s = slabs_[i];
+      if (s.free_.size() == 1 && s.free_.begin()->second == s.size) {
+        free_mem(s.base);
+        reserved_ -= s.size;
+        slabs_.erase(slabs_.begin() + i);
+      } else {
+        ++i;
+      }
+    }
+  }
+  void release_all() {
+    for (Slab& s : slabs_) free_mem(s.base);
+    slabs_.clear();
+    used_.clear();
+    reserved_ = in_use_ = 0;
+  }
+  size_t reserved() const { return reserved_; }
+  size_t in_use() const { return in_use_; }
+  int grows() const { return grows_; }
+  int slabs() const { return (int)slabs_.size(); }
+
+ private:
+  struct Slab {
+    uint8_t* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+  };
+  static size_t round(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; }
+  void free_mem(void* p) {
+    if (host_) (void)hipHostFree(p);
+    else (void)hipFree(p);
+  }
+  bool host_;
+  std::vector<Slab> slabs_;
+  std::map<uint8_t*, size_t> used_;
+  size_t reserved_ = 0, in_use_ = 0;
+  int grows_ = 0;
+};
+
+// The server's memory: device and pinned pools on one stream.  Work is fenced once per tick:
+// blocks handed back during a tick (arenas of retired runs, tables of a replaced batch,
+// staging copies, replaced graphs) are released when that tick's fence event completes.
+// Fences complete in order (one stream), so completion is a sequence number.
+class ServerMem {
+ public:
+  ServerMem() : dev(false), pin(true) {}
+  ~ServerMem() {
+    for (Fence& f : fences_) (void)hipEventDestroy(f.ev);
+    for (hipEvent_t e : spare_) (void)hipEventDestroy(e);
+    for (Fence& f : fences_) release(f.items);
+    release(pending_);
+  }
+  hipStream_t stream = nullptr;
+  size_t dev_cap = SIZE_MAX;          // bound on device slab bytes (the HBM budget + tables)
+  static constexpr size_t kDevSlab = (size_t)4 << 30;
+  static constexpr size_t kPinSlab = (size_t)64 << 20;
+  static constexpr size_t kTabSlab = (size_t)16 << 20;
+  SlabPool dev, pin;
+  SlabPool tab{false};  // batch tables (KBs per run): slabs of 16 MiB outside dev_cap
+
+  // Device block of `bytes`, growing the pool by a slab of max(bytes, 4 GiB) within dev_cap.
+  // Returns 0 with the block in *out; 1 when dev_cap leaves no room now (blocks come back as
+  // runs retire); 2 on a HIP error (*err).
+  int alloc_dev(size_t bytes, void** out, hipError_t* err) {
+    *out = dev.take(bytes);
+    if (*out) return 0;
+    const size_t need = bytes + 256;
+    if (dev.reserved() + need > dev_cap) {
+      if (dev.in_use() != 0) return 1;
+      dev.trim();  // nothing in use, no work queued on the pool: give back the empty slabs
+      if (dev.reserved() + need > dev_cap) return 1;
+    }
+    const size_t slab = std::max(need, std::min(kDevSlab, dev_cap - dev.reserved()));
+    *err = dev.grow(slab);
+    if (*err != hipSuccess) return 2;
+    *out = dev.take(bytes);
+    if (!*out) { *err = hipErrorOutOfMemory; return 2; }
+    return 0;
+  }
+  hipError_t alloc_tab(size_t bytes, void** out) {
+    *out = tab.take(bytes);
+    if (*out) return hipSuccess;
+    hipError_t e = tab.grow(std::max(bytes + 256, kTabSlab));
+    if (e != hipSuccess) return e;
+    *out = tab.take(bytes);
+    return *out ? hipSuccess : hipErrorOutOfMemory;
+  }
+  void* alloc_pin(size_t bytes) {
+    void* p = pin.take(bytes);
+    if (!p && pin.grow(std::max(bytes + 256, kPinSlab)) == hipSuccess) p = pin.take(bytes);
+    return p;
+  }
+  // Stream-ordered copy of host data (staged in pinned memory, released after the fence).
+  hipError_t upload(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return hipSuccess;
+    void* st = alloc_pin(bytes);
+    if (!st) return hipErrorOutOfMemory;
+    memcpy(st, src, bytes);
+    pending_.push_back(Item{Item::kPin, st, nullptr, nullptr});
+    return hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, stream);
+  }
+  void free_dev_later(void* p) { if (p) pending_.push_back(Item{Item::kDev, p, nullptr, nullptr}); }
+  void free_tab_later(void* p) { if (p) pending_.push_back(Item{Item::kTab, p, nullptr, nullptr}); }
+  void free_pin_later(void* p) { if (p) pending_.push_back(Item{Item::kPin, p, nullptr, nullptr}); }
+  void destroy_graph_later(hipGraphExec_t x, hipGraph_t g) {
+    if (x || g) pending_.push_back(Item{Item::kGraph, nullptr, x, g});
+  }
+  // Records the fence of the work enqueued so far; returns its sequence number.
+  hipError_t fence(int64_t* seq) {
+    hipEvent_t e;
+    if (!spare_.empty()) { e = spare_.back(); spare_.pop_back(); }
+    else {
+      hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+      if (r != hipSuccess) return r;
+    }
+    hipError_t r = hipEventRecord(e, stream);
+    if (r != hipSuccess) { spare_.push_back(e); return r; }
+    Fence f;
+    f.ev = e;
+    f.seq = ++issued_;
+    f.items.swap(pending_);
+    fences_.push_back(std::move(f));
+    if (seq) *seq = issued_;
+    return hipSuccess;
+  }
+  // Releases what completed fences held (no wait); returns the last completed sequence.
+  int64_t poll() {
+    while (!fences_.empty()) {
+      hipError_t q = hipEventQuery(fences_.front().ev);
+      if (q == hipErrorNotReady) break;
+      retire_front();
+    }
+    return done_;
+  }
+  // Waits for the oldest outstanding fence (used only when the server has nothing to enqueue).
+  hipError_t wait_oldest() {
+    if (fences_.empty()) return hipSuccess;
+    hipError_t e = hipEventSynchronize(fences_.front().ev);
+    if (e == hipSuccess) retire_front();
+    return e;
+  }
+  int64_t done() const { return done_; }
+  int64_t issued() const { return issued_; }
+  bool idle() const { return fences_.empty(); }
+  bool has_pending() const { return !pending_.empty(); }
+
+ private:
+  struct Item {
+    enum Kind { kDev, kTab, kPin, kGraph } kind;
+    void* p;
+    hipGraphExec_t gx;
+    hipGraph_t g;
+  };
+  struct Fence {
+    hipEvent_t ev = nullptr;
+    int64_t seq = 0;
+    std::vector<Item> items;
+  };
+  void release(std::vector<Item>& items) {
+    for (Item& it : items) {
+      if (it.kind == Item::kDev) dev.give(it.p);
+      else if (it.kind == Item::kTab) tab.give(it.p);
+      else if (it.kind == Item::kPin) pin.give(it.p);
+      else {
+        if (it.gx) (void)hipGraphExecDestroy(it.gx);
+        if (it.g) (void)hipGraphDestroy(it.g);
+      }
+    }
+    items.clear();
+  }
+  void retire_front() {
+    Fence& f = fences_.front();
+    release(f.items);
+    done_ = f.seq;
+    spare_.push_back(f.ev);
+    fences_.pop_front();
+  }
+  std::deque<Fence> fences_;
+  std::vector<Item> pending_;
+  std::vector<hipEvent_t> spare_;
+  int64_t issued_ = 0, done_ = 0;
+};
+
 struct RunHost {
   pa_params p{};
   int w = 1, N = 0, n_steps = 0, cap_pop = 0, cap_units = 0;
@@ -107,15 +373,17 @@ struct RunHost {
   bool adaptive_done = false;
   double deltaBeta = 0.0;
   size_t footprint = 0;
-  // device memory (one allocation carved into pieces)
+  // device memory (one allocation carved into pieces); a served run's arena belongs to the
+  // server's pool, which takes it back behind the run's last queued work
   void* arena = nullptr;
+  bool pooled = false;
   RunDesc desc{};
   std::vector<StepOut> out_h;
   RunState st_h{};
   bool results_valid = false;
 
   ~RunHost() {
-    if (arena) (void)hipFree(arena);
+    if (arena && !pooled) (void)hipFree(arena);
   }
   int blocks(int kind) const {
     switch (kind) {
@@ -157,7 +425,7 @@ std::map<TuneKey, TuneVal>& tune_cache() {
 }
 TuneKey tune_key(const pa_params& p) {
   int rb = 0;
-  while ((1 << (rb + 1)) <= p.Rinit) ++rb;  // floor(log2 Rinit)
+  while (((int64_t)1 << (rb + 1)) <= (int64_t)p.Rinit) ++rb;  // floor(log2 Rinit)
   return TuneKey{p.coding, p.coding == PA_MSC ? p.msbits : 1, p.L, rb};
 }
 
@@ -243,6 +511,8 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     return fail(PA_ERR_NBMAX, "Error: number of temperature steps exceeds nBmax=10000.");
   h.R0 = p.Rinit;
   const double margin = cap_sigma * std::sqrt((double)p.Rinit) + 64.0;
+  if ((double)p.Rinit + margin + 64.0 >= 2147483647.0)
+    return fail(PA_ERR_INVALID, "Rinit too large: population capacity must stay below 2^31");
   h.cap_pop = p.Rinit + (int)margin;
   if (p.coding == PA_SSC) {
     h.cap_units = h.cap_pop;
@@ -274,7 +544,9 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   if (p.coding == PA_SSC) {
     const int ntmax = ceil_div(Nh, 128);
     const int Ld2 = p.L / 2;
-    const bool period3 = 128 % Ld2 != 0 && 384 % Ld2 == 0 && (384 / Ld2) % 2 == 0;
+    // (only while Ld2 < 128: a stream's first site is then its only one in row 0, and its
+    // last its only one in row L - 1; L = 384 takes the general block loop)
+    const bool period3 = Ld2 < 128 && 128 % Ld2 != 0 && 384 % Ld2 == 0 && (384 / Ld2) % 2 == 0;
     if (aligned && period3 && nb > h.nt / 128) h.path = 6;
     else if (aligned && nb > h.nt / 128 && nb <= 3 * (h.nt / 128)) h.path = 1;
     else if (aligned && nb > 3 * (h.nt / 128) && 128 % (p.L / 2) == 0) h.path = 4;
@@ -387,11 +659,30 @@ void step_thresholds(const RunHost& h, double b, uint64_t* t) {
   }
 }
 
-// Allocate and upload a planned run's state (tables, initial RunState); no kernels.
-int materialize(RunHost& h) {
+constexpr int kWaitForMemory = -1;  // internal: the server pool has no room for now
+
+// Allocate and upload a planned run's state (tables, initial RunState); no kernels.  With a
+// server pool the arena comes from the pool and the uploads are stream-ordered copies on the
+// server stream (kWaitForMemory: no room in the pool until runs retire).
+int materialize(RunHost& h, ServerMem* mem = nullptr) {
   h.footprint = layout(h, nullptr);
-  HIPCHECK(hipMalloc(&h.arena, h.footprint));
+  if (mem) {
+    hipError_t err = hipSuccess;
+    const int r = mem->alloc_dev(h.footprint, &h.arena, &err);
+    if (r == 1) return kWaitForMemory;
+    if (r == 2) return fail(PA_ERR_HIP, std::string("GPUassert: ") + hipGetErrorString(err) +
+                                            " (server pool, " + std::to_string(h.footprint) + " bytes)");
+    h.pooled = true;
+  } else {
+    HIPCHECK(hipMalloc(&h.arena, h.footprint));
+  }
   layout(h, (uint8_t*)h.arena);
+  auto put = [&](void* dst, const void* src, size_t n) {
+    return mem ? mem->upload(dst, src, n) : hipMemcpy(dst, src, n, hipMemcpyHostToDevice);
+  };
+  auto zero = [&](void* dst, size_t n) {
+    return mem ? hipMemsetAsync(dst, 0, n, mem->stream) : hipMemset(dst, 0, n);
+  };
   RunDesc& d = h.desc;
   d.L = h.p.L; d.Ld2 = h.p.L / 2; d.N = h.N; d.Nh = h.N / 2;
   d.coding = h.p.coding; d.w = h.w; d.Rinit = h.p.Rinit; d.n_steps = h.n_steps;
@@ -410,12 +701,12 @@ int materialize(RunHost& h) {
     dbeta[i] = h.B[i] - h.B[i - 1];
     step_thresholds(h, h.B[i], &thr[3 * i]);
   }
-  HIPCHECK(hipMemcpy((void*)d.beta, beta.data(), (h.n_steps + 1) * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy((void*)d.dbeta, dbeta.data(), (h.n_steps + 1) * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy((void*)d.thr, thr.data(), thr.size() * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemset(d.out, 0, (h.n_steps + 1) * sizeof(StepOut)));
-  HIPCHECK(hipMemset(d.flags, 0, (size_t)h.cap_units * 8 * 4));
-  if (d.hist) HIPCHECK(hipMemset(d.hist, 0, (size_t)(h.N + 1) * 4));
+  HIPCHECK(put((void*)d.beta, beta.data(), (h.n_steps + 1) * 8));
+  HIPCHECK(put((void*)d.dbeta, dbeta.data(), (h.n_steps + 1) * 8));
+  HIPCHECK(put((void*)d.thr, thr.data(), thr.size() * 8));
+  HIPCHECK(zero(d.out, (h.n_steps + 1) * sizeof(StepOut)));
+  HIPCHECK(zero(d.flags, (size_t)h.cap_units * 8 * 4));
+  if (d.hist) HIPCHECK(zero(d.hist, (size_t)(h.N + 1) * 4));
   RunState st{};
   st.step = 1;
   st.R = h.p.Rinit;
@@ -428,7 +719,7 @@ int materialize(RunHost& h) {
   st.overflow = 0;
   st.err = 0;
   st.active = h.n_steps > 0 ? 1 : 0;
-  HIPCHECK(hipMemcpy(d.st, &st, sizeof(RunState), hipMemcpyHostToDevice));
+  HIPCHECK(put(d.st, &st, sizeof(RunState)));
   h.st_h = st;
   return PA_OK;
 }
@@ -538,12 +829,23 @@ struct Batch {
   double sweep_ms = 0.0, resample_ms = 0.0, total_ms = 0.0;
   int timed_steps = 0;
   std::vector<void*> table_allocs;
+  // serving: tables come from the server's pool and are released behind its fences, the
+  // uploads are stream-ordered; null for plain runs and batches (hipMalloc at creation)
+  ServerMem* mem = nullptr;
+  bool no_side = false;  // a batch that only launches initial populations: no side streams
 
   ~Batch() { release(); }
   void release_tables() {
-    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
-    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
-    for (void* p : table_allocs) (void)hipFree(p);
+    if (mem) {
+      mem->destroy_graph_later(gexec, graph);
+      gexec = nullptr;
+      graph = nullptr;
+      for (void* p : table_allocs) mem->free_tab_later(p);
+    } else {
+      if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+      if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+      for (void* p : table_allocs) (void)hipFree(p);
+    }
     table_allocs.clear();
     groups.clear();
     for (RsGroup& g : rs_groups) g = RsGroup{};
@@ -564,9 +866,13 @@ struct Batch {
   }
 
   int alloc_table(void** p, size_t bytes) {
-    HIPCHECK(hipMalloc(p, bytes));
+    if (mem) HIPCHECK(mem->alloc_tab(bytes, p));
+    else HIPCHECK(hipMalloc(p, bytes));
     table_allocs.push_back(*p);
     return PA_OK;
+  }
+  hipError_t put(void* dst, const void* src, size_t bytes) {
+    return mem ? mem->upload(dst, src, bytes) : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   }
 
   // Build descriptor and prefix tables for the current run set.
@@ -611,9 +917,9 @@ struct Batch {
     if ((rc = alloc_table((void**)&offs_d, offs.size() * 4))) return rc;
     if ((rc = alloc_table((void**)&is0_d, n * 8))) return rc;
     if ((rc = alloc_table((void**)&pause_d, n * 4))) return rc;
-    HIPCHECK(hipMemcpy(desc_d, descs.data(), n * sizeof(RunDesc), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(offs_d, offs.data(), offs.size() * 4, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(is0_d, is0.data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHECK(put(desc_d, descs.data(), n * sizeof(RunDesc)));
+    HIPCHECK(put(offs_d, offs.data(), offs.size() * 4));
+    HIPCHECK(put(is0_d, is0.data(), n * 8));
     // sweep groups (by kernel variant); the init launches use the same grouping
     {
       std::map<SweepVariant, std::vector<int>> gm;
@@ -647,11 +953,11 @@ struct Batch {
         g.lds = lds;
         if ((rc = alloc_table((void**)&g.desc_d, m * sizeof(RunDesc)))) return rc;
         if ((rc = alloc_table((void**)&g.offs_d, go.size() * 4))) return rc;
-        HIPCHECK(hipMemcpy(g.desc_d, gd.data(), m * sizeof(RunDesc), hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(g.offs_d, go.data(), go.size() * 4, hipMemcpyHostToDevice));
+        HIPCHECK(put(g.desc_d, gd.data(), m * sizeof(RunDesc)));
+        HIPCHECK(put(g.offs_d, go.data(), go.size() * 4));
         uint64_t* gis_d = nullptr;
         if ((rc = alloc_table((void**)&gis_d, m * 8))) return rc;
-        HIPCHECK(hipMemcpy(gis_d, gis.data(), m * 8, hipMemcpyHostToDevice));
+        HIPCHECK(put(gis_d, gis.data(), m * 8));
         plan_balance(g, runs);
         groups.push_back(g);
         init_is0.push_back(gis_d);
@@ -682,10 +988,10 @@ struct Batch {
       g.blocks = acc;
       if ((rc = alloc_table((void**)&g.desc_d, g.m * sizeof(RunDesc)))) return rc;
       if ((rc = alloc_table((void**)&g.offs_d, go.size() * 4))) return rc;
-      HIPCHECK(hipMemcpy(g.desc_d, gd.data(), g.m * sizeof(RunDesc), hipMemcpyHostToDevice));
-      HIPCHECK(hipMemcpy(g.offs_d, go.data(), go.size() * 4, hipMemcpyHostToDevice));
+      HIPCHECK(put(g.desc_d, gd.data(), g.m * sizeof(RunDesc)));
+      HIPCHECK(put(g.offs_d, go.data(), go.size() * 4));
     }
-    return ensure_side_streams();
+    return no_side ? PA_OK : ensure_side_streams();
   }
   std::vector<uint64_t*> init_is0;
 
@@ -975,12 +1281,8 @@ struct Batch {
 };
 
 // Copy step records and state back; compute the results rows like SSC:620-630 and 679-690.
-int fetch(RunHost& h, hipStream_t s) {
-  h.out_h.resize(h.n_steps + 1);
-  HIPCHECK(hipMemcpyAsync(h.out_h.data(), h.desc.out, (h.n_steps + 1) * sizeof(StepOut),
-                          hipMemcpyDeviceToHost, s));
-  HIPCHECK(hipMemcpyAsync(&h.st_h, h.desc.st, sizeof(RunState), hipMemcpyDeviceToHost, s));
-  HIPCHECK(hipStreamSynchronize(s));
+// Error bits of a run's state once its step records and state are on the host.
+int check_fetched(RunHost& h) {
   if (h.st_h.err)
     return fail(PA_ERR_HIP, "balanced sweep hand-over wait timed out (device error bits " +
                                 std::to_string(h.st_h.err) + ")");
@@ -988,6 +1290,15 @@ int fetch(RunHost& h, hipStream_t s) {
     return fail(PA_ERR_CAPACITY, "population outgrew its buffers");
   h.results_valid = true;
   return PA_OK;
+}
+
+int fetch(RunHost& h, hipStream_t s) {
+  h.out_h.resize(h.n_steps + 1);
+  HIPCHECK(hipMemcpyAsync(h.out_h.data(), h.desc.out, (h.n_steps + 1) * sizeof(StepOut),
+                          hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(&h.st_h, h.desc.st, sizeof(RunState), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  return check_fetched(h);
 }
 
 int rows_of(const RunHost& h, pa_row* rows, int max_rows, int* nrows) {
@@ -1061,7 +1372,15 @@ int pa_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes
   int dev = 0;
   HIPCHECK(hipGetDevice(&dev));
   HIPCHECK(hipGetDeviceProperties(&prop, dev));
-  if (name && name_len) snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  if (name && name_len) {
+    // some drivers leave the marketing name empty: describe the device by architecture,
+    // compute units and engine clock instead
+    if (prop.name[0])
+      snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    else
+      snprintf(name, name_len, "AMD %s, %d CUs, %d MHz", prop.gcnArchName,
+               prop.multiProcessorCount, prop.clockRate / 1000);
+  }
   if (cu_count) *cu_count = prop.multiProcessorCount;
   if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
   return PA_OK;
@@ -1718,11 +2037,17 @@ int pa_batch_destroy(pa_batch* b) {
 }  // extern "C"
 
 // ---------------- server: continuous batching under an HBM budget ----------------
-// Requests wait in a FIFO queue; at every tick the server retires finished runs, admits
-// queued runs while their footprint fits the budget (and the resident-run limit), and then
-// advances every resident run by the same number of steps with one batched launch
-// sequence.  Runs that join later simply start at their own step 1: every kernel reads its
-// run's own state, so a run's results do not depend on which other runs share the device.
+// Requests wait in a FIFO queue.  Each tick picks up the results of runs whose last step
+// has completed on the device, admits queued runs while their footprint fits the budget (and
+// the resident-run limit), and advances the scheduled resident runs with one batched launch
+// sequence.  Runs that join later start at their own step 1: every kernel reads its run's
+// own state, so a run's results do not depend on which other runs share the device.
+//
+// Nothing on the tick path allocates device memory or drains the device: arenas and tables
+// come from the server's pools (ServerMem), uploads are stream-ordered copies from pinned
+// memory, and a run whose last step has been enqueued has its step records copied into
+// pinned memory behind that step; a later tick finds the copy complete (an event query) and
+// only then returns the run's arena to the pool.
 namespace {
 
 struct Request {
@@ -1730,12 +2055,15 @@ struct Request {
   pa_params p{};
   size_t footprint = 0;
   std::unique_ptr<RunHost> h;
-  int state = 0;  // 0 queued, 1 resident, 2 done, 3 failed
+  int state = 0;  // 0 queued, 1 resident, 2 done, 3 failed, 4 last step enqueued (draining)
   int steps_left = 0;
   double credit = 0;  // serving: step credit carried between ticks
   int rc = PA_OK;
+  std::string err;
   std::vector<pa_row> rows;
   double t_submit = 0, t_admit = 0, t_done = 0;
+  void* pin = nullptr;  // draining: step records + state, copied behind fence `seq`
+  int64_t seq = 0;
 };
 
 double now_s() {
@@ -1747,18 +2075,20 @@ double now_s() {
 struct pa_server {
   size_t budget = 0;
   int max_runs = 0;
-  size_t used = 0;
+  size_t used = 0;   // footprints of the admitted runs whose arenas are not back in the pool
+  ServerMem mem;     // before `batch`: the batch hands its tables back to these pools
   Batch batch;
   std::deque<int64_t> queue;
   std::map<int64_t, Request> reqs;
   std::vector<int64_t> resident;
+  std::vector<int64_t> draining;
   int64_t next_ticket = 1;
   int completed = 0;
   int chunk = 32;
   int policy = PA_SERVE_SHORTEST;
   bool dirty = false;
   std::vector<int32_t> pause, quota;
-  hipEvent_t tick_ev[2] = {nullptr, nullptr};  // end of the last two ticks' device work
+  int64_t tick_seq[2] = {0, 0};  // fences of the last two ticks that enqueued work
   int64_t ticks = 0;
 };
 
@@ -1771,9 +2101,27 @@ int pa_server_create(size_t hbm_budget_bytes, int max_runs, pa_server** out) {
   s->max_runs = max_runs;
   int rc = s->batch.create_stream();
   if (rc) return rc;
-  for (hipEvent_t& e : s->tick_ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  s->mem.stream = s->batch.stream;
+  // arenas only; the per-run footprints sum to at most the budget, fragmentation aside
+  s->mem.dev_cap = hbm_budget_bytes + (size_t)256 * 4096;
+  s->batch.mem = &s->mem;
   s->batch.use_graph = true;
   *out = s.release();
+  return PA_OK;
+}
+
+int pa_server_reserve(pa_server* s, size_t bytes) {
+  if (!s) return fail(PA_ERR_INVALID, "null server");
+  bytes = std::min(bytes, s->mem.dev_cap);
+  if (s->mem.dev.reserved() >= bytes) return PA_OK;
+  HIPCHECK(s->mem.dev.grow(bytes - s->mem.dev.reserved()));
+  return PA_OK;
+}
+
+int pa_server_memory(pa_server* s, size_t* reserved, int* device_allocations) {
+  if (!s) return fail(PA_ERR_INVALID, "null server");
+  if (reserved) *reserved = s->mem.dev.reserved() + s->mem.tab.reserved();
+  if (device_allocations) *device_allocations = s->mem.dev.grows() + s->mem.tab.grows();
   return PA_OK;
 }
 
@@ -1810,84 +2158,128 @@ int pa_server_submit(pa_server* s, const pa_params* p, int64_t* ticket) {
   return PA_OK;
 }
 
-static int server_retire(pa_server* s) {
-  bool any = false;
-  for (int64_t t : s->resident) any |= s->reqs[t].steps_left == 0;
-  if (!any) return PA_OK;
-  HIPCHECK(hipStreamSynchronize(s->batch.stream));
+}  // extern "C"
+
+namespace {
+
+// Results of the draining runs whose copies have completed (no wait).
+void server_collect(pa_server* s) {
+  const int64_t done = s->mem.poll();
+  if (s->draining.empty()) return;
   const double tnow = now_s();
   std::vector<int64_t> keep;
-  for (int64_t t : s->resident) {
+  for (int64_t t : s->draining) {
     Request& rq = s->reqs[t];
-    if (rq.steps_left > 0) { keep.push_back(t); continue; }
-    rq.rc = fetch(*rq.h, s->batch.stream);
+    if (rq.seq > done) { keep.push_back(t); continue; }
+    RunHost& h = *rq.h;
+    h.out_h.resize(h.n_steps + 1);
+    memcpy(h.out_h.data(), rq.pin, (h.n_steps + 1) * sizeof(StepOut));
+    memcpy(&h.st_h, (const uint8_t*)rq.pin + (h.n_steps + 1) * sizeof(StepOut), sizeof(RunState));
+    s->mem.pin.give(rq.pin);
+    rq.pin = nullptr;
+    rq.rc = check_fetched(h);
     if (rq.rc == PA_OK) {
-      rq.rows.resize(rq.h->n_steps + 1);
+      rq.rows.resize(h.n_steps + 1);
       int n = 0;
-      rows_of(*rq.h, rq.rows.data(), (int)rq.rows.size(), &n);
+      rows_of(h, rq.rows.data(), (int)rq.rows.size(), &n);
       rq.rows.resize(n);
       rq.state = 2;
     } else {
+      rq.err = g_err;
       rq.state = 3;
     }
     rq.t_done = tnow;
-    s->used -= rq.footprint;
+    s->used -= rq.footprint;  // the arena went back to the pool with fence rq.seq
     rq.h.reset();
     s->completed++;
-    s->dirty = true;
   }
-  s->resident.swap(keep);
-  return PA_OK;
+  s->draining.swap(keep);
 }
 
-static int server_admit(pa_server* s) {
+// A request that cannot be admitted (its arena or uploads failed): it fails alone.
+void server_reject(pa_server* s, Request& rq, int rc) {
+  rq.rc = rc;
+  rq.err = g_err;
+  rq.state = 3;
+  rq.t_done = now_s();
+  if (rq.h && rq.h->arena && rq.h->pooled) s->mem.free_dev_later(rq.h->arena);
+  rq.h.reset();
+  s->completed++;
+}
+
+// Admit queued runs while they fit; returns whether any were admitted.  The initial
+// populations of the newcomers are launched on the server stream.
+int server_admit(pa_server* s, bool* admitted) {
   std::vector<RunHost*> fresh;
+  *admitted = false;
   while (!s->queue.empty() && (int)s->resident.size() < s->max_runs) {
     Request& rq = s->reqs[s->queue.front()];
     if (s->used + rq.footprint > s->budget) break;
-    int rc = materialize(*rq.h);
-    if (rc) return rc;
+    const int rc = materialize(*rq.h, &s->mem);
+    if (rc == kWaitForMemory) break;  // pool fragments: wait for retirements
+    s->queue.pop_front();
+    if (rc != PA_OK) { server_reject(s, rq, rc); continue; }
     s->used += rq.footprint;
     rq.state = 1;
     rq.steps_left = rq.h->n_steps;
     rq.t_admit = now_s();
     s->resident.push_back(rq.ticket);
     fresh.push_back(rq.h.get());
-    s->queue.pop_front();
     s->dirty = true;
   }
-  if (!fresh.empty()) {
-    // initial populations of the newcomers, on the server stream
-    Batch init;
-    init.stream = s->batch.stream;
-    init.runs = fresh;
-    int rc = init.build_tables();
-    if (!rc) rc = init.launch_init();
-    hipError_t e = hipStreamSynchronize(s->batch.stream);
-    init.stream = nullptr;  // borrowed
-    if (rc) return rc;
-    HIPCHECK(e);
-  }
+  if (fresh.empty()) return PA_OK;
+  *admitted = true;
+  Batch init;
+  init.stream = s->batch.stream;  // borrowed
+  init.mem = &s->mem;
+  init.no_side = true;
+  init.runs = fresh;
+  int rc = init.build_tables();
+  if (!rc) rc = init.launch_init();
+  init.stream = nullptr;
+  return rc;
+}
+
+// Step records and state of a run whose last step has just been enqueued, copied into
+// pinned memory behind it; its arena returns to the pool behind the same fence.
+int server_drain(pa_server* s, Request& rq) {
+  RunHost& h = *rq.h;
+  const size_t rec = (h.n_steps + 1) * sizeof(StepOut);
+  rq.pin = s->mem.alloc_pin(rec + sizeof(RunState));
+  if (!rq.pin) return fail(PA_ERR_HIP, "pinned staging memory for a finished run");
+  HIPCHECK(hipMemcpyAsync(rq.pin, h.desc.out, rec, hipMemcpyDeviceToHost, s->batch.stream));
+  HIPCHECK(hipMemcpyAsync((uint8_t*)rq.pin + rec, h.desc.st, sizeof(RunState),
+                          hipMemcpyDeviceToHost, s->batch.stream));
+  s->mem.free_dev_later(h.arena);
+  h.arena = nullptr;
+  rq.state = 4;
   return PA_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int pa_server_tick(pa_server* s, int* completed) {
   if (!s) return fail(PA_ERR_INVALID, "null server");
   int rc;
   // At most two ticks in flight: the device always has the next tick queued, and the host
   // sees arrivals and schedules within a tick of the device instead of queueing whole
-  // schedules ahead of it.
-  if (s->ticks >= 2) HIPCHECK(hipEventSynchronize(s->tick_ev[s->ticks & 1]));
-  if ((rc = server_retire(s))) return rc;
-  if ((rc = server_admit(s))) return rc;
+  // schedules ahead of it.  This waits for the tick before last, never for the last one.
+  if (s->ticks >= 2)
+    while (s->mem.done() < s->tick_seq[s->ticks & 1]) HIPCHECK(s->mem.wait_oldest());
+  server_collect(s);
+  bool admitted = false;
+  if ((rc = server_admit(s, &admitted))) return rc;
   if (s->dirty) {
     s->batch.runs.clear();
     for (int64_t t : s->resident) s->batch.runs.push_back(s->reqs[t].h.get());
     if ((rc = s->batch.build_tables())) return rc;
     s->dirty = false;
   }
+  bool enqueued = admitted;
   if (!s->resident.empty()) {
-    // Equal device time per resident run (processor sharing): run i earns c_max / c_i
+    // Equal device time per scheduled run (processor sharing): run i earns c_max / c_i
     // steps of credit per tick, c = EqSw * N * Rinit its work per step, and runs the whole
     // part of its credit (at most `chunk`); it pauses on the device after its quota while
     // the tick replays the step max_i k_i times.  Cheap runs therefore finish in few ticks
@@ -1936,31 +2328,58 @@ int pa_server_tick(pa_server* s, int* completed) {
       s->pause[i] = rq.h->n_steps - rq.steps_left + k;  // k = 0: paused for this tick
       K = std::max(K, k);
     }
-    HIPCHECK(hipMemcpyAsync(s->batch.pause_d, s->pause.data(), n * sizeof(int32_t),
-                            hipMemcpyHostToDevice, s->batch.stream));
+    HIPCHECK(s->mem.upload(s->batch.pause_d, s->pause.data(), n * sizeof(int32_t)));
     HIPCHECK(pa_launch_resume(s->batch.stream, s->batch.desc_d, (int)n, s->batch.pause_d));
     if ((rc = s->batch.run_steps(K))) return rc;
-    HIPCHECK(hipEventRecord(s->tick_ev[s->ticks & 1], s->batch.stream));
-    ++s->ticks;
-    for (size_t i = 0; i < n; ++i) s->reqs[s->resident[i]].steps_left -= s->quota[i];
-    if ((rc = server_retire(s))) return rc;
+    enqueued = true;
+    // runs whose last step is now enqueued leave the batch; their results follow the step
+    std::vector<int64_t> keep;
+    for (size_t i = 0; i < n; ++i) {
+      const int64_t t = s->resident[i];
+      Request& rq = s->reqs[t];
+      rq.steps_left -= s->quota[i];
+      if (rq.steps_left > 0) { keep.push_back(t); continue; }
+      if ((rc = server_drain(s, rq))) return rc;
+      s->draining.push_back(t);
+      s->dirty = true;
+    }
+    s->resident.swap(keep);
   }
+  if (enqueued || s->mem.has_pending()) {
+    int64_t seq = 0;
+    HIPCHECK(s->mem.fence(&seq));
+    for (int64_t t : s->draining)
+      if (s->reqs[t].seq == 0) s->reqs[t].seq = seq;
+    if (enqueued) {
+      s->tick_seq[s->ticks & 1] = seq;
+      ++s->ticks;
+    }
+  }
+  if (!enqueued && !s->mem.idle()) {
+    // nothing to enqueue (no run resident, none admissible): wait for the device's oldest
+    // fence instead of spinning; it returns finished runs and their memory
+    HIPCHECK(s->mem.wait_oldest());
+  }
+  server_collect(s);
   if (completed) *completed = s->completed;
   return PA_OK;
 }
 
 int pa_server_poll(pa_server* s, int64_t ticket, int* done) {
   if (!s || !done) return fail(PA_ERR_INVALID, "null argument");
+  server_collect(s);
   auto it = s->reqs.find(ticket);
   if (it == s->reqs.end()) return fail(PA_ERR_INVALID, "unknown ticket");
-  *done = it->second.state >= 2 ? 1 : 0;
-  return it->second.state == 3 ? it->second.rc : PA_OK;
+  *done = it->second.state == 2 || it->second.state == 3 ? 1 : 0;
+  if (it->second.state == 3) return fail(it->second.rc, it->second.err);
+  return PA_OK;
 }
 
 int pa_server_results(pa_server* s, int64_t ticket, pa_row* rows, int max_rows, int* nrows) {
   if (!s || !rows || !nrows) return fail(PA_ERR_INVALID, "null argument");
   auto it = s->reqs.find(ticket);
   if (it == s->reqs.end()) return fail(PA_ERR_INVALID, "unknown ticket");
+  if (it->second.state == 3) return fail(it->second.rc, it->second.err);
   if (it->second.state != 2) return fail(PA_ERR_STATE, "request not finished");
   const int n = std::min((int)it->second.rows.size(), max_rows);
   memcpy(rows, it->second.rows.data(), n * sizeof(pa_row));
@@ -1979,8 +2398,9 @@ int pa_server_latency(pa_server* s, int64_t ticket, double* submit_to_done_s, do
 
 int pa_server_stats(pa_server* s, int* queued, int* resident, size_t* hbm_used) {
   if (!s) return fail(PA_ERR_INVALID, "null server");
+  server_collect(s);
   if (queued) *queued = (int)s->queue.size();
-  if (resident) *resident = (int)s->resident.size();
+  if (resident) *resident = (int)(s->resident.size() + s->draining.size());
   if (hbm_used) *hbm_used = s->used;
   return PA_OK;
 }
@@ -1988,8 +2408,6 @@ int pa_server_stats(pa_server* s, int* queued, int* resident, size_t* hbm_used) 
 int pa_server_destroy(pa_server* s) {
   if (!s) return PA_OK;
   (void)hipStreamSynchronize(s->batch.stream);
-  for (hipEvent_t e : s->tick_ev)
-    if (e) (void)hipEventDestroy(e);
   s->batch.runs.clear();
   delete s;
   return PA_OK;

@@ -48,3 +48,50 @@ def compare_rows(orows, erows, check_hash=True):
         for f in ("E", "C", "M", "M2", "M4", "BF", "S"):
             assert getattr(o, f).hex() == getattr(e, f).hex(), (ctx, f, getattr(o, f), getattr(e, f))
     return len(orows)
+
+
+def load_golden(name):
+    """A committed oracle trace file tests/golden/oracle_<name>.json."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"oracle_{name}.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def params_of(d, **over):
+    """Engine parameters of a trace's parameter record."""
+    kw = dict(d)
+    kw.update(over)
+    return pa.make_params(**kw)
+
+
+def compare_trace(trace, erows, check_hash=True):
+    """Assert that the engine's rows agree bit for bit with the first len(trace) records of
+    an oracle trace (R, q, lnQ and FP64 sums as hex, integer sums, hashes, Philox counter)."""
+    assert len(erows) >= len(trace), (len(erows), len(trace))
+    for t, e in zip(trace, erows):
+        ctx = f"step {t['i']}"
+        assert t["R"] == e.R, (ctx, "R", t["R"], e.R)
+        if t["i"] == 0:
+            continue
+        assert t["Rprev"] == e.Rprev, (ctx, "Rprev")
+        assert t["q"] == e.q.hex(), (ctx, "q", t["q"], e.q.hex())
+        assert t["lnQ"] == e.lnQ.hex(), (ctx, "lnQ")
+        assert t["sums"] == [x.hex() for x in e.sums], (ctx, "sums")
+        assert (t["sumE"], t["sumM"]) == (e.sumE, e.sumM), (ctx, "integer sums")
+        if check_hash:
+            assert t["roff_hash"] == e.roff_hash, (ctx, "roff_hash")
+            assert t["spin_hash"] == e.spin_hash, (ctx, "spin_hash")
+        assert t["IS"] == e.IS_after, (ctx, "IS")
+    return len(trace)
+
+
+def same_rows(a, b):
+    """Bit-exact equality of two engine row lists (batched or served against plain)."""
+    assert len(a) == len(b), (len(a), len(b))
+    for i, (x, y) in enumerate(zip(a, b)):
+        for f in ("R", "Rprev", "sumE", "sumM", "roff_hash", "spin_hash", "IS_after"):
+            assert getattr(x, f) == getattr(y, f), (i, f)
+        for f in ("q", "lnQ", "E", "C", "M", "M2", "M4", "BF", "S"):
+            assert getattr(x, f).hex() == getattr(y, f).hex(), (i, f)

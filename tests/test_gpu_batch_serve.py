@@ -1,33 +1,32 @@
-"""Batched, graph-replayed and served runs are bit-exact to the same run on the plain path."""
+"""Batched, graph-replayed and served runs are bit-exact to the oracle and to the same run on
+the plain path."""
 import random
 
 import pytest
 
 import paising_amd as pa
-from tests.helpers import engine_params
+from tests.helpers import compare_rows, engine_params, oracle_rows, same_rows
 
 pytestmark = pytest.mark.gpu
 
-
-def same(a, b):
-    assert len(a) == len(b)
-    for i, (x, y) in enumerate(zip(a, b)):
-        for f in ("R", "Rprev", "sumE", "sumM", "roff_hash", "spin_hash", "IS_after"):
-            assert getattr(x, f) == getattr(y, f), (i, f)
-        for f in ("q", "lnQ", "E", "C", "M", "M2", "M4", "BF", "S"):
-            assert getattr(x, f).hex() == getattr(y, f).hex(), (i, f)
+same = same_rows
 
 
-def mix(n, seed):
+def mix_cases(n, seed):
+    """(L, coding, msbits, Rinit, EqSw, dB, Bfin, seed, steps) of a small mixed run list."""
     rng = random.Random(seed)
     out = []
     for k in range(n):
         L = rng.choice([16, 24, 32, 48, 64])
         coding = rng.choice([pa.SSC, pa.MSC])
         w = rng.choice([32, 64]) if coding == pa.MSC else 32
-        out.append(engine_params(L, coding, w, rng.randint(100, 1500), rng.choice([2, 5, 10]),
-                                 rng.choice([0.01, 0.02]), 1.0, 100 + k, rng.randint(3, 9)))
+        out.append((L, coding, w, rng.randint(100, 1500), rng.choice([2, 5, 10]),
+                    rng.choice([0.01, 0.02]), 1.0, 100 + k, rng.randint(3, 9)))
     return out
+
+
+def mix(n, seed):
+    return [engine_params(*c) for c in mix_cases(n, seed)]
 
 
 def test_graph_replay_matches_plain():
@@ -44,19 +43,23 @@ def test_graph_replay_matches_plain():
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_matches_plain(use_graph):
-    plist = mix(12, 7)
+def test_batch_matches_oracle_and_plain(use_graph):
+    cases = mix_cases(12, 7)
+    plist = [engine_params(*c) for c in cases]
     b = pa.Batch(plist, use_graph=use_graph)
     b.run()
-    for i, p in enumerate(plist):
+    for i, (c, p) in enumerate(zip(cases, plist)):
+        got = b.results(i)
+        compare_rows(oracle_rows(*c), got)
         plain, _ = pa.run_all(p)
-        same(plain, b.results(i))
+        same(plain, got)
     b.close()
 
 
 @pytest.mark.parametrize("policy", [pa.SERVE_SHORTEST, pa.SERVE_SHARE])
-def test_server_matches_plain_alone_and_in_mix(policy):
-    plist = mix(10, 11)
+def test_server_matches_oracle_and_plain_alone_and_in_mix(policy):
+    cases = mix_cases(10, 11)
+    plist = [engine_params(*c) for c in cases]
     srv = pa.Server(hbm_budget_bytes=64 * 2**20, max_runs=4, steps_per_tick=3, policy=policy)
     tickets = []
     for i, p in enumerate(plist):
@@ -64,13 +67,62 @@ def test_server_matches_plain_alone_and_in_mix(policy):
         srv.tick()            # requests arrive while others are on the device
     while not all(srv.done(t) for t in tickets):
         srv.tick()
-    for t, p in zip(tickets, plist):
+    for t, c, p in zip(tickets, cases, plist):
+        got = srv.results(t)
+        compare_rows(oracle_rows(*c), got)
         plain, _ = pa.run_all(p)
-        same(plain, srv.results(t))
+        same(plain, got)
         a, b = srv.latency(t)
         assert a >= b >= 0
     q, res, used = srv.stats()
     assert q == 0 and res == 0 and used == 0
+    srv.close()
+
+
+def test_server_ticks_allocate_no_device_memory_once_reserved():
+    """With the pool reserved up front, admitting and retiring a stream of requests makes no
+    further device allocation, and every result stays bit-exact to the plain path."""
+    cases = mix_cases(14, 23)
+    plist = [engine_params(*c) for c in cases]
+    srv = pa.Server(hbm_budget_bytes=256 * 2**20, max_runs=6, steps_per_tick=2)
+    srv.reserve(256 * 2**20)
+    warm = srv.submit(plist[0])            # first tick creates the table pool and graph
+    while not srv.done(warm):
+        srv.tick()
+    reserved, allocs = srv.memory()
+    assert reserved >= 256 * 2**20
+    tickets = []
+    for p in plist:
+        tickets.append(srv.submit(p))
+        srv.tick()
+    while not all(srv.done(t) for t in tickets):
+        srv.tick()
+    assert srv.memory() == (reserved, allocs)
+    for t, p in zip(tickets, plist):
+        same(pa.run_all(p)[0], srv.results(t))
+    srv.close()
+
+
+def test_server_request_that_cannot_be_created_fails_alone():
+    """A request whose device state cannot be allocated (about 2 TB: more than the device
+    holds, though within the stated budget) fails by itself; the requests around it run and
+    stay bit-exact, and the queue moves on."""
+    small = [engine_params(32, pa.MSC, 32, 800, 3, 0.02, 1.0, 61 + i, 4) for i in range(2)]
+    huge = pa.make_params(L=64, coding=pa.MSC, msbits=64, Rinit=2_000_000_000, eqsweeps=1,
+                          dB=0.01, Bfin=1.0, seed=3, max_steps=1)
+    assert pa.footprint(huge) > 1.5e12
+    srv = pa.Server(hbm_budget_bytes=4 * 10**12, max_runs=8, steps_per_tick=2)
+    t0 = srv.submit(small[0])
+    th = srv.submit(huge)
+    t1 = srv.submit(small[1])
+    for _ in range(10000):
+        if srv.done(t0) and srv.done(t1):
+            break
+        srv.tick()
+    with pytest.raises(pa.PAError):
+        srv.done(th)
+    for t, p in zip((t0, t1), small):
+        same(pa.run_all(p)[0], srv.results(t))
     srv.close()
 
 
