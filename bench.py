@@ -14,8 +14,13 @@ resampling kernel's HBM roofline ("roofline_resample"), the same run in single-s
 ("ssc") and a CPU baseline (the oracle, C++/OpenMP, on this host's cores) over a bounded
 sample.  Kernel times come from HIP events recorded on the run's stream around each step's
 resampling and sweep launches, over as many steps again right after the throughput steps
-(the events themselves idle the device for a few microseconds per step).  --mode batch / serve measure the batched (64-run mix of uneven shapes) and served
-(Poisson arrivals) paths instead.
+(the events themselves idle the device for a few microseconds per step).
+
+The other two parts of the metric ride on the same line: "batched" (the 64-run mix of
+uneven shapes run as one batch against the plain path one run at a time, runs/s and
+bit-exactness) and "served" (Poisson arrivals at 20 requests/s: p50/p99 latency, runs/s and
+how far the offered work exceeds what the device completes; then the same request shapes at
+60 % of the measured capacity).  --mode batch / serve print those alone.
 """
 import argparse
 import json
@@ -28,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import paising_amd as pa  # noqa: E402
+from paising_amd.workloads import attempts, batch_mix, serve_requests, small_mix  # noqa: E402
 
 L, RINIT, EQSW, DB, SEED = 64, 20000, 100, 0.005, 100
 N = L * L
@@ -50,7 +56,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload (FETCH_SIZE scaled x2, WRITE_SIZE x1, the factors a plain device copy of known
 # size reads back): profiles/k_sweep_msc_bal_msc32_L64_R20000.md, k_resample_L64_R20000.md.
+# They are constants of those profiles, not measured in this run (counters need their own
+# rocprofv3 pass); "traffic_source" in the JSON says so.
 TRAFFIC_BYTES = {"sweep_msc32": 37.63e6, "resample_msc32": 21.41e6}
+TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this workload (profile constant)"
 
 
 def msc_ops_per_attempt(w=32):
@@ -82,13 +91,23 @@ def cpu_threads():
 
 
 def cpu_model():
+    """Model name, family/model numbers and logical CPU count from /proc/cpuinfo (the model
+    name alone can be a bare vendor string on virtual machines)."""
+    info = {}
+    ncpu = 0
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k == "processor":
+                ncpu += 1
+            elif k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                info[k] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    return (f"{info.get('model name', 'unknown')} (family {info.get('cpu family', '?')}, "
+            f"model {info.get('model', '?')}, stepping {info.get('stepping', '?')}; "
+            f"{ncpu} logical CPUs visible)")
 
 
 def oracle_rate(coding, sample_steps, threads):
@@ -212,33 +231,9 @@ def bench_adaptive(args):
     return out
 
 
-def batch_mix(nruns=64, seed=2026, steps=0):
-    """The batch mix: 64 independent runs of uneven L, R and EqSw, both codings, dB=0.005, beta 0->1."""
-    import random
-    rng = random.Random(seed)
-    plist = []
-    for k in range(nruns):
-        Lk = rng.choice([16, 24, 32, 48, 64, 96])
-        Rk = rng.randint(1000, 50000)
-        eq = rng.choice([10, 50, 100])
-        coding = rng.choice([pa.SSC, pa.MSC])
-        plist.append(pa.make_params(L=Lk, coding=coding, msbits=32, Rinit=Rk, eqsweeps=eq, dB=DB,
-                                    Bfin=1.0, seed=100 + k, max_steps=steps))
-    return plist
-
-
-def small_mix(nruns=64, seed=7):
-    """Latency-bound mix: 64 small runs (L in {16,24,32}, R in [500,2000], EqSw=10, SSC/MSC32),
-    each too small to fill the device alone."""
-    import random
-    rng = random.Random(seed)
-    return [pa.make_params(L=rng.choice([16, 24, 32]), coding=rng.choice([pa.SSC, pa.MSC]),
-                           msbits=32, Rinit=rng.randint(500, 2000), eqsweeps=10, dB=DB, Bfin=1.0,
-                           seed=500 + k) for k in range(nruns)]
-
-
 def batch_vs_plain(plist):
-    """(batched wall s, plain one-at-a-time wall s, bit-exact, beta steps) for one run list."""
+    """Batched vs plain one-at-a-time on one run list: wall seconds of each, bit-exactness
+    of every batched run against its plain counterpart, beta steps and spin-flip attempts."""
     t0 = time.perf_counter()
     b = pa.Batch(plist, use_graph=True)
     b.run()
@@ -250,65 +245,56 @@ def batch_vs_plain(plist):
     t_plain = time.perf_counter() - t0
     exact = all(pa.format_rows(x, 17) == pa.format_rows(y, 17) and
                 [r.spin_hash for r in x] == [r.spin_hash for r in y] for x, y in zip(brows, prows))
-    return t_batch, t_plain, exact, sum(len(r) - 1 for r in brows)
+    att = sum(attempts(p, r) for p, r in zip(plist, brows))
+    return dict(t_batch=t_batch, t_plain=t_plain, exact=exact,
+                steps=sum(len(r) - 1 for r in brows), attempts=att)
 
 
-def bench_batch(args):
-    """Batched path (one launch sequence per step for all runs, hipGraph replay) against the
-    plain path launched one run at a time, on the same runs; every batched run is checked
-    bit-exact against its plain counterpart.  Also reported: the same comparison on a
-    latency-bound mix of small runs, where batching, not the kernels, sets the rate."""
+def batched_summary(nruns=64, batch_steps=0, small=True):
+    """The batch mix (64 runs, L in {16,...,96}, R in [1000,50000]) batched and plain; with
+    small=True also the latency-bound small-run mix."""
     # one-time process costs (code-object load, kernel attributes, first allocations) stay
     # out of both timings
     batch_vs_plain(small_mix(2, seed=99))
-    ts, tp, ex, st = batch_vs_plain(small_mix(args.nruns))
-    small = {"runs_per_s": round(args.nruns / ts, 3), "plain_runs_per_s": round(args.nruns / tp, 3),
-             "speedup_vs_plain": round(tp / ts, 3), "bit_exact_vs_plain": ex, "beta_steps_total": st,
-             "workload": f"{args.nruns} runs, L in {{16,24,32}}, R in [500,2000], EqSw=10, "
-                         "SSC/MSC32 mixed, dB=0.005, beta 0->1"}
-    plist = batch_mix(args.nruns, steps=args.batch_steps)
-    t_batch, t_plain, exact, steps = batch_vs_plain(plist)
-    return {
-        "metric": "batched PA runs/s (64-run mix) vs plain one-at-a-time",
-        "value": round(len(plist) / t_batch, 4), "unit": "runs/s", "n_gpus": 1,
-        "higher_is_better": True, "plain_runs_per_s": round(len(plist) / t_plain, 4),
-        "speedup_vs_plain": round(t_plain / t_batch, 3), "bit_exact_vs_plain": exact,
-        "wall_s_batched": round(t_batch, 3), "wall_s_plain": round(t_plain, 3),
-        "beta_steps_total": steps, "small_run_mix": small,
-        "config": {"workload": f"{len(plist)} runs, L in {{16,24,32,48,64,96}}, R in [1000,50000], "
-                               f"EqSw in {{10,50,100}}, SSC/MSC32 mixed, dB=0.005, beta 0->1"
-                               + (f", truncated to {args.batch_steps} steps" if args.batch_steps else "")},
-    }
-
-
-def serve_requests(n, seed=5):
-    """Served load: Poisson arrivals (20/s), L in {16..256 step 16}, R log-uniform in [1e3,1e5],
-    dB in {0.001, 0.005}, Bfin uniform in [0.44, 1.0], EqSw=50, MSC32."""
-    import random
-    rng = random.Random(seed)
-    t = 0.0
-    out = []
-    for k in range(n):
-        t += rng.expovariate(20.0)
-        L = 16 * rng.randint(1, 16)
-        R = int(round(10 ** rng.uniform(3, 5)))
-        out.append((t, pa.make_params(L=L, coding=pa.MSC, msbits=32, Rinit=R, eqsweeps=50,
-                                      dB=rng.choice([0.001, 0.005]), Bfin=rng.uniform(0.44, 1.0),
-                                      seed=1000 + k)))
+    plist = batch_mix(nruns, steps=batch_steps)
+    m = batch_vs_plain(plist)
+    out = {"runs_per_s": round(nruns / m["t_batch"], 4),
+           "plain_runs_per_s": round(nruns / m["t_plain"], 4),
+           "speedup_vs_plain": round(m["t_plain"] / m["t_batch"], 3),
+           "bit_exact_vs_plain": m["exact"], "wall_s_batched": round(m["t_batch"], 3),
+           "wall_s_plain": round(m["t_plain"], 3), "beta_steps_total": m["steps"],
+           "attempts_per_ns_batched": round(m["attempts"] / (m["t_batch"] * 1e9), 1),
+           "workload": f"{nruns} runs, L in {{16,24,32,48,64,96}}, R in [1000,50000], "
+                       "EqSw in {10,50,100}, SSC/MSC32 mixed, dB=0.005, beta 0->1"
+                       + (f", truncated to {batch_steps} steps" if batch_steps else "")}
+    if small:
+        ms = batch_vs_plain(small_mix(nruns))
+        out["small_run_mix"] = {
+            "runs_per_s": round(nruns / ms["t_batch"], 3),
+            "plain_runs_per_s": round(nruns / ms["t_plain"], 3),
+            "speedup_vs_plain": round(ms["t_plain"] / ms["t_batch"], 3),
+            "bit_exact_vs_plain": ms["exact"], "beta_steps_total": ms["steps"],
+            "workload": f"{nruns} runs, L in {{16,24,32}}, R in [500,2000], EqSw=10, "
+                        "SSC/MSC32 mixed, dB=0.005, beta 0->1"}
     return out
 
 
-def bench_serve(args):
-    reqs = serve_requests(args.nreq)
-    # one-time process costs (code-object load, first allocations) stay out of the latencies
+def warm_server():
+    """One tiny request through a throw-away server: the process's one-time costs
+    (code-object load, first allocations) stay out of the latencies."""
     warm = pa.Server(hbm_budget_bytes=1 << 30, max_runs=4)
     tw = warm.submit(pa.make_params(L=16, coding=pa.MSC, msbits=32, Rinit=500, eqsweeps=5,
                                     dB=0.05, Bfin=0.5, seed=1))
     while not warm.done(tw):
         warm.tick()
     warm.close()
-    policy = {"shortest": pa.SERVE_SHORTEST, "share": pa.SERVE_SHARE}[args.policy]
-    srv = pa.Server(hbm_budget_bytes=200 * 10**9, max_runs=256, steps_per_tick=args.tick_steps,
+
+
+def serve_load(reqs, policy=pa.SERVE_SHORTEST, tick_steps=32, check=4):
+    """Submit each request at its arrival time (seconds after the start) to a server with a
+    200 GB budget, tick until all are done; latency = submission to completion.  `check`
+    of the cheapest requests are rerun on the plain path and compared bit for bit."""
+    srv = pa.Server(hbm_budget_bytes=200 * 10**9, max_runs=256, steps_per_tick=tick_steps,
                     policy=policy)
     t0 = time.perf_counter()
     pending = list(reqs)
@@ -318,29 +304,81 @@ def bench_serve(args):
         while pending and pending[0][0] <= now:
             _, p = pending.pop(0)
             tickets.append((srv.submit(p), p))
-        if tickets:
+        if tickets and not all(srv.done(t) for t, _ in tickets):
             srv.tick()
-        else:
-            time.sleep(min(0.001, max(0.0, pending[0][0] - now)))
+        elif pending:
+            time.sleep(min(0.002, max(0.0, pending[0][0] - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
     lat = sorted(srv.latency(t)[0] for t, _ in tickets)
+    rows = {t: srv.results(t) for t, _ in tickets}
+    att = sum(attempts(p, rows[t]) for t, p in tickets)
+    _, allocs = srv.memory()
+
     def pct(q):
         return lat[min(len(lat) - 1, int(math.ceil(q * len(lat))) - 1)]
     exact = True
-    if args.check:
-        for t, p in tickets[: args.check]:
-            exact &= pa.format_rows(srv.results(t), 17) == pa.format_rows(pa.run_all(p)[0], 17)
+    cheapest = sorted(tickets, key=lambda tp: attempts(tp[1], rows[tp[0]]))[:check]
+    for t, p in cheapest:
+        exact &= pa.format_rows(rows[t], 17) == pa.format_rows(pa.run_all(p)[0], 17)
     srv.close()
-    return {
-        "metric": "served PA runs: p50/p99 completion latency (Poisson 20 req/s arrivals)",
-        "value": round(pct(0.5), 4), "unit": "s (p50 latency)", "p99_s": round(pct(0.99), 4),
-        "runs_per_s": round(len(lat) / wall, 4), "n_requests": len(lat), "n_gpus": 1,
-        "wall_s": round(wall, 3), "latencies_s": [round(x, 3) for x in lat],
-        "higher_is_better": False, "bit_exact_vs_plain_checked": args.check if exact else "FAILED",
-        "config": {"workload": "Poisson 20 req/s, L in {16..256 step 16}, R log-uniform [1e3,1e5], "
-                               "dB in {0.001,0.005}, Bfin U[0.44,1], EqSw=50, MSC32, 200 GB budget",
-                   "policy": args.policy, "steps_per_tick": args.tick_steps},
-    }
+    return dict(p50=pct(0.5), p99=pct(0.99), wall=wall, n=len(lat), lat=lat, attempts=att,
+                exact=exact, checked=len(cheapest), device_allocations=allocs)
+
+
+def served_summary(nreq=16, nreq_sustained=24, tick_steps=32, policy=pa.SERVE_SHORTEST,
+                   check=4):
+    """Served load at the stated arrival rate (20 requests/s) and at a rate the device
+    sustains.  The stated rate offers far more work than one device completes, so its p50/p99
+    describe a burst that queues; `offered_load_factor` says how far above capacity it is."""
+    warm_server()
+    burst = serve_requests(nreq)
+    b = serve_load(burst, policy, tick_steps, check)
+    mean_att = b["attempts"] / b["n"]
+    capacity = b["attempts"] / b["wall"]           # attempts/s with the device saturated
+    offered = 20.0 * mean_att                      # attempts/s the stated arrivals bring
+    out = {"p50_s": round(b["p50"], 3), "p99_s": round(b["p99"], 3),
+           "runs_per_s": round(b["n"] / b["wall"], 4), "n_requests": b["n"],
+           "arrival_rate_per_s": 20.0, "offered_load_factor": round(offered / capacity, 1),
+           "device_attempts_per_ns": round(capacity / 1e9, 1),
+           "mean_attempts_per_request": float("%.4g" % mean_att),
+           "wall_s": round(b["wall"], 2), "latencies_s": [round(x, 2) for x in b["lat"]],
+           "bit_exact_vs_plain_checked": b["checked"] if b["exact"] else "FAILED",
+           "server_device_allocations": b["device_allocations"],
+           "workload": "Poisson arrivals, L in {16..256 step 16}, R log-uniform [1e3,1e5], dB in "
+                       "{0.001,0.005}, Bfin U[0.44,1], EqSw=50, MSC32, 200 GB budget"}
+    if nreq_sustained:
+        # the same request shapes at 60 % of the measured capacity
+        rate = 0.6 * capacity / mean_att
+        s = serve_load(serve_requests(nreq_sustained, rate=rate), policy, tick_steps, check)
+        out["sustainable"] = {
+            "arrival_rate_per_s": round(rate, 4), "p50_s": round(s["p50"], 3),
+            "p99_s": round(s["p99"], 3), "runs_per_s": round(s["n"] / s["wall"], 4),
+            "n_requests": s["n"], "wall_s": round(s["wall"], 2),
+            "latencies_s": [round(x, 2) for x in s["lat"]],
+            "bit_exact_vs_plain_checked": s["checked"] if s["exact"] else "FAILED"}
+    return out
+
+
+def bench_batch(args):
+    """Batched path (one launch sequence per step for all runs, hipGraph replay) against the
+    plain path launched one run at a time, on the same runs; every batched run is checked
+    bit-exact against its plain counterpart.  Also reported: the same comparison on a
+    latency-bound mix of small runs, where batching, not the kernels, sets the rate."""
+    b = batched_summary(args.nruns, args.batch_steps)
+    out = {"metric": "batched PA runs/s (64-run mix) vs plain one-at-a-time",
+           "value": b.pop("runs_per_s"), "unit": "runs/s", "n_gpus": 1, "higher_is_better": True}
+    out.update(b)
+    return out
+
+
+def bench_serve(args):
+    policy = {"shortest": pa.SERVE_SHORTEST, "share": pa.SERVE_SHARE}[args.policy]
+    s = served_summary(args.nreq, args.nreq_sustained, args.tick_steps, policy, args.check)
+    out = {"metric": "served PA runs: p50/p99 completion latency (Poisson arrivals)",
+           "value": s["p50_s"], "unit": "s (p50 latency)", "n_gpus": 1, "higher_is_better": False,
+           "policy": args.policy, "steps_per_tick": args.tick_steps}
+    out.update(s)
+    return out
 
 
 def main():
@@ -358,6 +396,10 @@ def main():
     ap.add_argument("--nruns", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=0, help="truncate batched runs (0 = full)")
     ap.add_argument("--nreq", type=int, default=16)
+    ap.add_argument("--nreq-sustained", type=int, default=24,
+                    help="requests of the served run at a sustainable arrival rate (0 = none)")
+    ap.add_argument("--no-batch", action="store_true", help="default mode: skip the batched line")
+    ap.add_argument("--no-serve", action="store_true", help="default mode: skip the served line")
     ap.add_argument("--tick-steps", type=int, default=32)
     ap.add_argument("--policy", default="shortest", choices=["shortest", "share"],
                     help="server tick policy (serve mode)")
@@ -411,7 +453,7 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_sweep_msc (sweeps + energies)",
                      "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 3),
                      "unit": "Tlane-op/s", "frac": round(achieved / peak, 4),
-                     "traffic": TRAFFIC_BYTES["sweep_msc32"],
+                     "traffic": TRAFFIC_BYTES["sweep_msc32"], "traffic_source": TRAFFIC_SOURCE,
                      "ops_per_attempt": round(ops, 4),
                      "sweep_ms_per_step": round(m["sweep_ms"] / m["ev_steps"], 4),
                      "sweep_attempts_per_ns": round(m["ev_attempts"] / (m["sweep_ms"] * 1e6), 2)},
@@ -419,6 +461,7 @@ def main():
                               "achieved": round(rs_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(rs_gbs / HBM_PEAK_GBS, 4),
                               "traffic": TRAFFIC_BYTES["resample_msc32"],
+                              "traffic_source": TRAFFIC_SOURCE,
                               "resample_ms_per_step": round(m["resample_ms"] / m["ev_steps"], 4)},
     }
     if not args.no_ssc:
@@ -432,6 +475,12 @@ def main():
                       "valu_frac": round(s_ach / peak, 4),
                       "resample_GBps": round(s_["rs_bytes"] / (s_["resample_ms"] * 1e-3) / 1e9, 1),
                       "variant": s_["variant"]}
+    if not args.no_batch:
+        out["batched"] = batched_summary(args.nruns, args.batch_steps)
+    if not args.no_serve:
+        policy = {"shortest": pa.SERVE_SHORTEST, "share": pa.SERVE_SHARE}[args.policy]
+        out["served"] = served_summary(args.nreq, args.nreq_sustained, args.tick_steps, policy,
+                                       args.check)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, cpu_threads())
         if "ssc" in out:
