@@ -170,6 +170,17 @@ int pa_mhr_analysis(int N, int nB, const double* B, double* BF, const int32_t* R
  * (unit B uses is + 128 B + t), then energies and magnetisations. */
 int pa_stage_sweep(const pa_params* p, int R, uint64_t is, double beta, void* pop,
                    int32_t* IE, int32_t* M);
+/* pa_stage_sweep on device memory, allocating nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  pop (the population, layout as above), IE and M
+ * (ceil(R/w)*w or R int32 each) are device pointers owned by the caller; the work is enqueued
+ * on `stream` (a hipStream_t, NULL = the null stream) and the call returns without waiting.
+ * The run's descriptor, state and thresholds go to a fixed 4096-byte device array of the
+ * library, one per device, kept until the process ends; makespan balancing is not used (it
+ * needs flags per unit), which leaves the results unchanged.  Calls that overlap on one
+ * device race on that array: order them on one stream or wait between them. */
+int pa_stage_sweep_noalloc(const pa_params* p, int R, uint64_t is, double beta, void* pop,
+                           int32_t* IE, int32_t* M, void* stream);
 /* energyKer alone (PAisingSSC.cu:195-215, PAisingMSC.cu:268-317). */
 int pa_stage_energy(const pa_params* p, int R, const void* pop, int32_t* IE, int32_t* M);
 /* QKer, lnQ, CalcTauKer and CalcParSum (PAisingSSC.cu:217-270,533; PAisingMSC.cu:319-395,

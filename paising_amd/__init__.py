@@ -99,6 +99,7 @@ SIGNATURES = {
     "pa_stage_averages": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "pa_server_latency": (_I, [_P, ctypes.c_int64, _P, _P]),
     "pa_server_reserve": (_I, [_P, _SZ]),
+    "pa_stage_sweep_noalloc": (_I, [_P, _I, ctypes.c_uint64, ctypes.c_double, _P, _P, _P, _P]),
     "pa_server_memory": (_I, [_P, _P, _P]),
     "pa_server_destroy": (_I, [_P]),
     "pa_run_footprint": (_SZ, [_P]),
@@ -186,6 +187,13 @@ def stage_sweep(params, pop, R, IS, beta):
     M = np.zeros_like(IE)
     _check(lib().pa_stage_sweep(ctypes.byref(params), R, IS, beta, _ptr(out), _ptr(IE), _ptr(M)))
     return out, IE, M
+
+
+def stage_sweep_noalloc(params, R, IS, beta, pop_ptr, ie_ptr, m_ptr, stream=0):
+    """pa_stage_sweep_noalloc on caller-owned device buffers (addresses as integers, e.g.
+    torch data_ptr()); enqueued on `stream` (hipStream_t as an integer), not waited for."""
+    _check(lib().pa_stage_sweep_noalloc(ctypes.byref(params), R, IS, beta, pop_ptr, ie_ptr,
+                                        m_ptr, stream))
 
 
 def stage_energy(params, pop, R):

@@ -17,6 +17,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -659,6 +660,20 @@ void step_thresholds(const RunHost& h, double b, uint64_t* t) {
   }
 }
 
+// The scalar fields of a planned run's descriptor (geometry, coding, schedule length,
+// stream seed, resampling shape); pointers are set by the caller.
+void desc_scalars(RunHost& h) {
+  RunDesc& d = h.desc;
+  d.L = h.p.L; d.Ld2 = h.p.L / 2; d.N = h.N; d.Nh = h.N / 2;
+  d.coding = h.p.coding; d.w = h.w; d.Rinit = h.p.Rinit; d.n_steps = h.n_steps;
+  d.cap_units = h.cap_units; d.umap = h.p.uniform; d.eqsweeps = h.p.eqsweeps;
+  d.trace = h.p.trace; d.q128 = 128 / d.Ld2; d.r128 = 128 % d.Ld2; d.seed = h.p.seed;
+  d.rs_sites = h.rs_sites;
+  d.rs_lp = h.rs_lp;
+  d.rs_split = h.rs_split;
+  d.ld2_magic = (uint32_t)((1ull << 32) / (uint64_t)d.Ld2 + 1);
+}
+
 constexpr int kWaitForMemory = -1;  // internal: the server pool has no room for now
 
 // Allocate and upload a planned run's state (tables, initial RunState); no kernels.  With a
@@ -677,6 +692,7 @@ int materialize(RunHost& h, ServerMem* mem = nullptr) {
     HIPCHECK(hipMalloc(&h.arena, h.footprint));
   }
   layout(h, (uint8_t*)h.arena);
+  desc_scalars(h);
   auto put = [&](void* dst, const void* src, size_t n) {
     return mem ? mem->upload(dst, src, n) : hipMemcpy(dst, src, n, hipMemcpyHostToDevice);
   };
@@ -684,14 +700,6 @@ int materialize(RunHost& h, ServerMem* mem = nullptr) {
     return mem ? hipMemsetAsync(dst, 0, n, mem->stream) : hipMemset(dst, 0, n);
   };
   RunDesc& d = h.desc;
-  d.L = h.p.L; d.Ld2 = h.p.L / 2; d.N = h.N; d.Nh = h.N / 2;
-  d.coding = h.p.coding; d.w = h.w; d.Rinit = h.p.Rinit; d.n_steps = h.n_steps;
-  d.cap_units = h.cap_units; d.umap = h.p.uniform; d.eqsweeps = h.p.eqsweeps;
-  d.trace = h.p.trace; d.q128 = 128 / d.Ld2; d.r128 = 128 % d.Ld2; d.seed = h.p.seed;
-  d.rs_sites = h.rs_sites;
-  d.rs_lp = h.rs_lp;
-  d.rs_split = h.rs_split;
-  d.ld2_magic = (uint32_t)((1ull << 32) / (uint64_t)d.Ld2 + 1);
   std::vector<double> beta(h.n_steps + 1, 0.0), dbeta(h.n_steps + 1, 0.0);
   std::vector<uint64_t> thr((size_t)(h.n_steps + 1) * 3, 0);
   beta[0] = h.B[0];
@@ -721,6 +729,16 @@ int materialize(RunHost& h, ServerMem* mem = nullptr) {
   st.active = h.n_steps > 0 ? 1 : 0;
   HIPCHECK(put(d.st, &st, sizeof(RunState)));
   h.st_h = st;
+  return PA_OK;
+}
+
+// Kernel attributes (dynamic LDS limits) set once per process before the first launch.
+int prepare_kernels_once() {
+  static bool prepared = false;
+  if (!prepared) {
+    HIPCHECK(pa_prepare_kernels());
+    prepared = true;
+  }
   return PA_OK;
 }
 
@@ -1268,11 +1286,8 @@ struct Batch {
   }
 
   int create_stream() {
-    static bool prepared = false;
-    if (!prepared) {
-      HIPCHECK(pa_prepare_kernels());
-      prepared = true;
-    }
+    int rc = prepare_kernels_once();
+    if (rc) return rc;
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&ev0));
     HIPCHECK(hipEventCreate(&ev1));
@@ -1971,6 +1986,127 @@ int pa_stage_averages(const pa_params* p, int R, const int32_t* IE, const int32_
 }
 
 }  // extern "C"
+
+// ---------------- allocation-free sweep stage ----------------
+// pa_stage_sweep_noalloc: the sweeps and energies of pa_stage_sweep on caller-owned device
+// memory, for callers that cannot allocate (code under HIP graph capture, serving loops).
+// The only device memory beyond the caller's population and energy arrays is the run's
+// descriptor, launch table, state and thresholds: a fixed NoAllocTables record in the
+// file-scope `scratch` array below (one copy per device, held until the process ends), written
+// by a setup kernel from its by-value arguments -- no copy or memset call -- and read by the
+// sweep launch that follows it on the same stream.  Calls that overlap on one device race on
+// that record.
+namespace {
+
+struct NoAllocTables {
+  RunDesc desc;
+  int32_t offs[kKindCount * 2];  // launch table of one run (only the per-unit row is read)
+  RunState st;
+  uint64_t thr[6];               // thresholds of step 1 at [3..5]
+  double beta[2];
+  double dbeta[2];
+};
+constexpr size_t kNoAllocScratchBytes = 4096;
+static_assert(sizeof(NoAllocTables) <= kNoAllocScratchBytes, "scratch record exceeds the array");
+static_assert(sizeof(NoAllocTables) % 4 == 0, "setup kernel copies 32-bit words");
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+// The array's device address per device (the host symbol only names it), taken once per
+// device with that device current.
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(PA_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(PA_ERR_HIP, "no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PA_OK;
+}
+
+__global__ __launch_bounds__(64) void k_noalloc_setup(NoAllocTables t, NoAllocTables* dst) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&t);
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  for (int i = threadIdx.x; i < (int)(sizeof(NoAllocTables) / 4); i += blockDim.x) d[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" int pa_stage_sweep_noalloc(const pa_params* p, int R, uint64_t is, double beta,
+                                      void* pop, int32_t* IE, int32_t* M, void* stream) {
+  if (!p || !pop || !IE || !M) return fail(PA_ERR_INVALID, "null argument");
+  if (R < 1) return fail(PA_ERR_INVALID, "R must be at least 1");
+  if (p->adaptive || p->mhr || p->schedule_len)
+    return fail(PA_ERR_INVALID, "the sweep stage takes no schedule, adaptive or MHR options");
+  pa_params q = *p;
+  q.Rinit = R;
+  q.Binit = 0.0;
+  q.dB = 1.0;
+  q.Bfin = 1.0;
+  q.max_steps = 1;
+  q.trace = 0;
+  q.schedule = nullptr;
+  // makespan-balanced sweeps hand units over through per-unit flags that would need
+  // scratch proportional to R; every sweep variant gives the same bits
+  q.balance = -1;
+  RunHost h;
+  int rc = plan_run(q, h, 10.0);
+  if (rc) return rc;
+  if (sizeof(NoAllocTables) > kNoAllocScratchBytes)
+    return fail(PA_ERR_INVALID, "scratch record exceeds the " + std::to_string(kNoAllocScratchBytes) +
+                                    "-byte array");
+  void* ptr = nullptr;
+  if ((rc = noalloc_scratch(&ptr))) return rc;
+  if ((rc = prepare_kernels_once())) return rc;
+  NoAllocTables* dev = reinterpret_cast<NoAllocTables*>(ptr);
+  const int units = h.p.coding == PA_SSC ? R : ceil_div(R, h.w);
+  h.cap_units = units;
+  desc_scalars(h);
+  NoAllocTables t;
+  memset(&t, 0, sizeof t);
+  t.desc = h.desc;
+  t.desc.pop[0] = t.desc.pop[1] = pop;
+  t.desc.IE = IE;
+  t.desc.M = M;
+  t.desc.st = &dev->st;
+  t.desc.thr = dev->thr;
+  t.desc.beta = dev->beta;
+  t.desc.dbeta = dev->dbeta;
+  t.offs[kKUnits * 2 + 0] = 0;
+  t.offs[kKUnits * 2 + 1] = units;
+  // step 1 of a run whose sweep streams start at `is` (no parents: R = 0), reading the
+  // population from buffer buf ^ 1 = 0, as pa_stage_sweep sets it up
+  t.st.step = 1;
+  t.st.R = 0;
+  t.st.Rnew = R;
+  t.st.buf = 1;
+  t.st.IS = is;
+  t.st.active = 1;
+  step_thresholds(h, beta, &t.thr[3]);
+  t.beta[1] = beta;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(64), 0, s, t, dev);
+  HIPCHECK(hipGetLastError());
+  const size_t lds = h.p.coding == PA_SSC ? (size_t)h.N
+                     : (size_t)(h.half ? h.N / 2 : h.N) * (h.sw / 8);
+  if (h.p.coding == PA_MSC && h.half)
+    HIPCHECK(pa_launch_step(16, PA_MSC, h.w, 32, 1024, units * (h.w / 32), lds, s, &dev->desc,
+                            dev->offs, 1));
+  else
+    HIPCHECK(pa_launch_step(4, h.p.coding, h.w, h.p.coding == PA_SSC ? h.path : h.sw, h.nt,
+                            units * (h.p.coding == PA_SSC ? 1 : h.w / h.sw), lds, s, &dev->desc,
+                            dev->offs, 1));
+  return PA_OK;
+}
 
 // ---------------- batch ----------------
 int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out) {
