@@ -73,6 +73,7 @@ def lib():
             "ora_exp": (D, [D]),
             "ora_log": (D, [D]),
             "ora_block_reduce": (D, [P, I]),
+            "ora_small_block_reduce": (D, [P, I]),
             "ora_ssc_init": (None, [I, I, U64, U64, I, P]),
             "ora_ssc_sweep": (None, [I, I, P, U32, U64, U64, D, I]),
             "ora_ssc_energy": (None, [I, I, P, P, P]),

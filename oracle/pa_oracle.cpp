@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -237,6 +238,16 @@ double block_reduce(const double* vals, int n) {  // n = blockDim.x, multiple of
   return warp_tree(w);
 }
 
+/* smallblockReduceSum MSC:96-105 (blocks of fewer than 32 threads, power of 2): shared[t]
+ * += shared[t + stride] for stride = n/2, n/4, ..., 1; the result is shared[0]. */
+double small_block_reduce(const double* vals, int n) {
+  double s[32];
+  for (int i = 0; i < n; ++i) s[i] = vals[i];
+  for (int stride = n >> 1; stride > 0; stride >>= 1)
+    for (int t = 0; t < stride; ++t) s[t] += s[t + stride];
+  return s[0];
+}
+
 inline uint64_t mix64(uint64_t z) {  // splitmix64 finaliser (Steele, Lea, Flood 2014)
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -442,10 +453,11 @@ void ssc_averages(int R, const int32_t* IE, const int32_t* M, double out[5]) {
   }
 }
 
-/* ---------------- MSC stages (W = uint32_t for 32 replicas/word, uint64_t for 64) ---------------- */
+/* ---------------- MSC stages (W = uint8_t, uint16_t, uint32_t, uint64_t: MSbits 8/16/32/64) ---------------- */
 
-/* ReplicaInit MSC:169-188.  For 64 bits the two curand() calls of MSC:183-184 have no
- * specified order; the first word drawn becomes the high half (decision, unpinned). */
+/* ReplicaInit MSC:169-188.  8 and 16 bits keep the low byte / half of one curand() word
+ * (MSC:173-178), 32 bits the whole word.  For 64 bits the two curand() calls of MSC:183-184
+ * have no specified order; the first word drawn becomes the high half (decision, unpinned). */
 template <typename W>
 void msc_init(const Geo& g, int rg, int R, uint64_t seed, uint64_t IS, W* words, uint8_t* active) {
   const int w = (int)sizeof(W) * 8;
@@ -456,8 +468,8 @@ void msc_init(const Geo& g, int rg, int R, uint64_t seed, uint64_t IS, W* words,
     for (int t = 0; t < EQT; ++t) {
       Stream s(seed, IS + (uint64_t)t + (uint64_t)B * EQT);
       for (int idx = t; idx < g.Nh; idx += EQT) {
-        if (w == 32) {
-          gA[idx] = (W)s.next();
+        if (w <= 32) {
+          gA[idx] = (W)s.next();  // & 0xFF, & 0xFFFF or the whole word
           gB[idx] = (W)s.next();
         } else {
           uint64_t hi = s.next(); uint64_t lo = s.next();
@@ -550,8 +562,9 @@ void msc_energy(const Geo& g, int rg, const W* words, int32_t* IE, int32_t* M) {
   }
 }
 
-/* QKer MSC:319-345: lane sums per group (block of w threads), 1024-group block sums,
- * then the chunked final sum. */
+/* QKer MSC:319-345: lane sums per group (block of w threads: blockReduceSum for 32 and 64,
+ * smallblockReduceSum for 8 and 16, MSC:324-328), 1024-group block sums, then the chunked
+ * final sum. */
 double msc_q(int w, int rg, const uint8_t* active, const int32_t* IE, double dB, double Emean) {
   std::vector<double> gsum(rg);
   for (int gi = 0; gi < rg; ++gi) {
@@ -560,7 +573,7 @@ double msc_q(int w, int rg, const uint8_t* active, const int32_t* IE, double dB,
       size_t k = (size_t)gi * w + br;
       v[br] = active[k] ? my_exp(-dB * ((double)IE[k] - Emean)) : 0.0;
     }
-    gsum[gi] = block_reduce(v, w);
+    gsum[gi] = w < 32 ? small_block_reduce(v, w) : block_reduce(v, w);
   }
   int nb = (rg + NT - 1) / NT;
   std::vector<double> bsum(nb), v(NT);
@@ -1092,6 +1105,16 @@ int run_ssc(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, do
 }  // namespace
 
 /* ---------------- C API ---------------- */
+// word-type dispatch of the stage entry points: F is called with a W* null pointer whose type
+// selects the instantiation
+template <typename F>
+void by_width(int w, F f) {
+  if (w == 8) f((uint8_t*)nullptr);
+  else if (w == 16) f((uint16_t*)nullptr);
+  else if (w == 32) f((uint32_t*)nullptr);
+  else f((uint64_t*)nullptr);
+}
+
 extern "C" {
 
 int ora_run(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows, double* elapsed_s,
@@ -1110,7 +1133,9 @@ int ora_run_ex(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows,
   mo.hist = hist;
   mo.mhr = mhr;
   if (!p || p->L < 2 || (p->L & 1) || p->Rinit < 1 || p->runs < 1) return -1;
-  if (p->coding == ORA_MSC && p->msbits != 32 && p->msbits != 64) return -1;
+  if (p->coding == ORA_MSC && p->msbits != 8 && p->msbits != 16 && p->msbits != 32 &&
+      p->msbits != 64)
+    return -1;
   ora_params pp = *p;
   if (pp.adaptive && pp.min_overlap == 0.0 && pp.max_overlap == 0.0) {
     pp.min_overlap = 0.85;  // SSC:41
@@ -1123,6 +1148,8 @@ int ora_run_ex(const ora_params* p, ora_row* rows, int max_rows, int32_t* nrows,
   double tot = 0;
   int rc;
   if (p->coding == ORA_SSC) rc = run_ssc(p, rows, max_rows, nrows, &tot, mo);
+  else if (p->msbits == 8) rc = run_msc<uint8_t>(p, rows, max_rows, nrows, &tot, mo);
+  else if (p->msbits == 16) rc = run_msc<uint16_t>(p, rows, max_rows, nrows, &tot, mo);
   else if (p->msbits == 32) rc = run_msc<uint32_t>(p, rows, max_rows, nrows, &tot, mo);
   else rc = run_msc<uint64_t>(p, rows, max_rows, nrows, &tot, mo);
   auto t1 = std::chrono::steady_clock::now();
@@ -1149,6 +1176,7 @@ double ora_uniform(uint32_t x, uint32_t y, int umap) {
 double ora_exp(double x) { return my_exp(x); }
 double ora_log(double x) { return my_log(x); }
 double ora_block_reduce(const double* v, int n) { return block_reduce(v, n); }
+double ora_small_block_reduce(const double* v, int n) { return small_block_reduce(v, n); }
 
 uint64_t ora_hash_bytes(uint64_t h, const void* data, uint64_t n) {
   return h + hash_chunks((const uint8_t*)data, n, 1);
@@ -1168,19 +1196,25 @@ void ora_ssc_energy(int L, int R, const int8_t* spins, int32_t* IE, int32_t* M) 
 }
 void ora_msc_init(int L, int w, int rg, int R, uint64_t seed, uint64_t IS, void* words,
                   uint8_t* active) {
-  if (w == 32) msc_init<uint32_t>(Geo(L), rg, R, seed, IS, (uint32_t*)words, active);
-  else msc_init<uint64_t>(Geo(L), rg, R, seed, IS, (uint64_t*)words, active);
+  by_width(w, [&](auto* t) {
+    using W = std::remove_pointer_t<decltype(t)>;
+    msc_init<W>(Geo(L), rg, R, seed, IS, (W*)words, active);
+  });
 }
 void ora_msc_sweep(int L, int w, int rg, void* words, uint32_t sweeps, uint64_t seed,
                    uint64_t IS, double beta) {
   uint32_t T[2];
   msc_table(beta, T);
-  if (w == 32) msc_sweep<uint32_t>(Geo(L), rg, (uint32_t*)words, sweeps, seed, IS, T);
-  else msc_sweep<uint64_t>(Geo(L), rg, (uint64_t*)words, sweeps, seed, IS, T);
+  by_width(w, [&](auto* t) {
+    using W = std::remove_pointer_t<decltype(t)>;
+    msc_sweep<W>(Geo(L), rg, (W*)words, sweeps, seed, IS, T);
+  });
 }
 void ora_msc_energy(int L, int w, int rg, const void* words, int32_t* IE, int32_t* M) {
-  if (w == 32) msc_energy<uint32_t>(Geo(L), rg, (const uint32_t*)words, IE, M);
-  else msc_energy<uint64_t>(Geo(L), rg, (const uint64_t*)words, IE, M);
+  by_width(w, [&](auto* t) {
+    using W = std::remove_pointer_t<decltype(t)>;
+    msc_energy<W>(Geo(L), rg, (const W*)words, IE, M);
+  });
 }
 int64_t ora_ssc_tau(int R, const int32_t* IE, int Rinit, double dB, double lnQ, uint64_t seed,
                     uint64_t IS, int umap, uint32_t* Roff) {
@@ -1219,12 +1253,11 @@ void ora_ssc_resample(int L, int Rold, const int8_t* spins, const uint32_t* Roff
 void ora_msc_resample(int L, int w, int rg_old, const void* words, const uint8_t* active,
                       const int32_t* IE, const uint32_t* Roff, int Rnew, void* words_new,
                       uint8_t* active_new, int32_t* IE_new) {
-  if (w == 32)
-    msc_resample<uint32_t>(Geo(L), rg_old, (const uint32_t*)words, active, IE, Roff, Rnew,
-                           (uint32_t*)words_new, active_new, IE_new);
-  else
-    msc_resample<uint64_t>(Geo(L), rg_old, (const uint64_t*)words, active, IE, Roff, Rnew,
-                           (uint64_t*)words_new, active_new, IE_new);
+  by_width(w, [&](auto* t) {
+    using W = std::remove_pointer_t<decltype(t)>;
+    msc_resample<W>(Geo(L), rg_old, (const W*)words, active, IE, Roff, Rnew, (W*)words_new,
+                    active_new, IE_new);
+  });
 }
 void ora_ssc_table(double beta, double T[10]) { ssc_table(beta, T); }
 void ora_msc_table(double beta, uint32_t T[2]) { msc_table(beta, T); }

@@ -176,7 +176,8 @@ def _pop_shape(params, R):
     N = params.L * params.L
     if params.coding == SSC:
         return (R, N), np.int8
-    return (-(-R // params.msbits), N), (np.uint32 if params.msbits == 32 else np.uint64)
+    return (-(-R // params.msbits), N), {8: np.uint8, 16: np.uint16, 32: np.uint32,
+                                         64: np.uint64}[params.msbits]
 
 
 def stage_sweep(params, pop, R, IS, beta):
@@ -388,7 +389,7 @@ class Run:
         return v.value
 
     def population(self):
-        """SSC: int8 array (R, N); MSC: uint32/uint64 array (groups, N)."""
+        """SSC: int8 array (R, N); MSC: uint8/16/32/64 array (groups, N)."""
         nb = ctypes.c_size_t(0)
         _check(lib().pa_run_population(self.h, None, ctypes.byref(nb)))
         N = self.params.L * self.params.L
@@ -396,7 +397,7 @@ class Run:
             a = np.empty(nb.value, dtype=np.int8)
         else:
             a = np.empty(nb.value // (self.params.msbits // 8),
-                         dtype=np.uint32 if self.params.msbits == 32 else np.uint64)
+                         dtype=_pop_shape(self.params, 1)[1])
         _check(lib().pa_run_population(self.h, _ptr(a), ctypes.byref(nb)))
         return a.reshape(-1, N)
 

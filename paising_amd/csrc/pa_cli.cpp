@@ -4,7 +4,7 @@
 // codes as the reference main() (PAisingSSC.cu:338-806, PAisingMSC.cu:503-957), with the
 // compile-time constants made runtime options:
 //   -L size   (PAisingSSC.cu:27, default 64)
-//   -w bits   (PAisingMSC.cu:34 MSbits, 32 or 64, default 32; MSC only)
+//   -w bits   (PAisingMSC.cu:34 MSbits, 8, 16, 32 or 64, default 32; MSC only)
 //   -u conv   uniform-double convention: 0 = CUDA >= 10.1 (default), 1 = CUDA <= 10.0
 //   -a        adaptive inverse-temperature step (the reference's AdaptiveStep build,
 //             PAisingSSC.cu:23,40-43,406-410,644-660); later -M runs replay run 0's steps
@@ -87,7 +87,7 @@ void usage(const char* prog) {
             << " -o dataDirectory   ( dataDirectory   = data directory name )\n"
             << " -L size            ( size            = linear lattice size, even )\n"
 #if PA_CLI_CODING == 1
-            << " -w bits            ( bits            = replicas per multi-spin word, 32 or 64 )\n"
+            << " -w bits            ( bits            = replicas per multi-spin word, 8, 16, 32 or 64 )\n"
 #endif
             << " -u convention      ( convention      = uniform-double mapping: 0 CUDA>=10.1, 1 CUDA<=10.0 )\n"
             << " -a                 ( adaptive inverse temperature step, MinOverlap 0.85, MaxOverlap 0.87 )\n"

@@ -469,8 +469,8 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   }
   if (p.L < 4 || (p.L & 1) || p.L > 1024) return fail(PA_ERR_INVALID, "L must be even, 4..1024");
   if (p.coding != PA_SSC && p.coding != PA_MSC) return fail(PA_ERR_INVALID, "coding");
-  if (p.coding == PA_MSC && p.msbits != 32 && p.msbits != 64)
-    return fail(PA_ERR_INVALID, "msbits must be 32 or 64");
+  if (p.coding == PA_MSC && p.msbits != 8 && p.msbits != 16 && p.msbits != 32 && p.msbits != 64)
+    return fail(PA_ERR_INVALID, "msbits must be 8, 16, 32 or 64");
   if (p.Rinit < 1) return fail(PA_ERR_INVALID, "Rinit < 1");
   if (!(p.dB > 0.0) || p.Binit < 0.0) return fail(PA_ERR_INVALID, "dB must be > 0, Binit >= 0");
   if (p.uniform != PA_UNIFORM_CUDA101 && p.uniform != PA_UNIFORM_CUDA100)
@@ -773,7 +773,8 @@ int device_cus() {
 // count so that a share always exceeds one unit (a unit is cut at most once).
 void plan_balance(Group& g, const std::vector<RunHost*>& runs) {
   g.bal_workers = 0;
-  if (g.v.coding != PA_MSC || g.v.nt > 512) return;  // no balanced 1024-thread kernel
+  // no balanced 1024-thread kernel, and none for 8- and 16-bit words
+  if (g.v.coding != PA_MSC || g.v.nt > 512 || g.v.w < 32) return;
   int64_t units = 0;
   int force = 0;
   bool fast = true;
@@ -821,12 +822,16 @@ struct Batch {
   uint64_t* is0_d = nullptr;
   int32_t* pause_d = nullptr;  // serving: last step of the tick per run
   int blocks[kKindCount] = {0, 0, 0, 0, 0};
-  size_t rs_lds[2] = {0, 0};  // MSC resampling LDS for w = 32, 64
-  int rs_nt[2] = {256, 256};  // MSC resampling threads per workgroup for w = 32, 64
-  bool has_w[2] = {false, false};
+  // MSC word widths 8, 16, 32, 64 -> index 0..3
+  static constexpr int kWidths = 4;
+  static int width_index(int w) { return w == 8 ? 0 : w == 16 ? 1 : w == 32 ? 2 : 3; }
+  static int width_of(int wi) { return 8 << wi; }
+  size_t rs_lds[kWidths] = {0, 0, 0, 0};  // MSC resampling LDS per word width
+  int rs_nt[kWidths] = {256, 256, 256, 256};  // MSC resampling threads per workgroup
+  bool has_msc = false;
   std::vector<Group> groups;
-  static constexpr int kRsKinds = 5;
-  struct RsGroup {  // the runs of one resampling kernel (SSC, MSC32, MSC64, windowed 32 / 64)
+  static constexpr int kRsKinds = 1 + 2 * kWidths;
+  struct RsGroup {  // the runs of one resampling kernel (SSC; per width plain and windowed)
     RunDesc* desc_d = nullptr;
     int32_t* offs_d = nullptr;
     int m = 0, blocks = 0;
@@ -913,18 +918,20 @@ struct Batch {
       offs[(size_t)k * (n + 1) + n] = acc;
       blocks[k] = acc;
     }
-    rs_lds[0] = rs_lds[1] = 0;
-    rs_nt[0] = rs_nt[1] = 256;
-    has_w[0] = has_w[1] = false;
+    for (int wi = 0; wi < kWidths; ++wi) {
+      rs_lds[wi] = 0;
+      rs_nt[wi] = 256;
+    }
+    has_msc = false;
     for (int r = 0; r < n; ++r) {
       descs[r] = runs[r]->desc;
       is0[r] = runs[r]->p.initial_sequence;
-      if (runs[r]->p.coding == PA_MSC) has_w[runs[r]->w == 32 ? 0 : 1] = true;
+      if (runs[r]->p.coding == PA_MSC) has_msc = true;
       if (runs[r]->p.coding == PA_MSC && !runs[r]->rs_win) {
-        const int wi = runs[r]->w == 32 ? 0 : 1;
+        const int wi = width_index(runs[r]->w);
         rs_lds[wi] = std::max(rs_lds[wi], runs[r]->resample_lds());
         // 1024-thread resampling workgroups pay off for 32-bit populations above 256 groups
-        if (wi == 0 && runs[r]->cap_units > 256) rs_nt[0] = 1024;
+        if (runs[r]->w == 32 && runs[r]->cap_units > 256) rs_nt[wi] = 1024;
       }
       any_trace |= runs[r]->p.trace != 0;
       any_mhr |= runs[r]->p.mhr != 0;
@@ -981,10 +988,10 @@ struct Batch {
         init_is0.push_back(gis_d);
       }
     }
-    // resampling tables per kernel (SSC, MSC32, MSC64, windowed MSC32 and MSC64): each launch
-    // covers only the runs its kernel serves, not the blocks of the other runs
+    // resampling tables per kernel (SSC; MSC per word width, plain and windowed): each
+    // launch covers only the runs its kernel serves, not the blocks of the other runs
     auto rs_kind = [](const RunHost& h) {
-      return h.p.coding == PA_SSC ? 0 : (h.w == 32 ? 1 : 2) + (h.rs_win ? 2 : 0);
+      return h.p.coding == PA_SSC ? 0 : 1 + width_index(h.w) + (h.rs_win ? kWidths : 0);
     };
     for (int k = 0; k < kRsKinds; ++k) {
       RsGroup& g = rs_groups[k];
@@ -1063,7 +1070,7 @@ struct Batch {
   // q partials, lnQ and copy counts (QKer, CalcTauKer).
   int launch_counts() {
     const int n = (int)runs.size();
-    if (has_w[0] || has_w[1])
+    if (has_msc)
       HIPCHECK(pa_launch_step(8, 0, 0, 0, 0, blocks[kKChunks], 0, stream, desc_d, offs_d, n));
     if (has_ssc())
       HIPCHECK(pa_launch_step(0, 0, 0, 0, 0, blocks[kKGroupChunks], 0, stream, desc_d, offs_d, n));
@@ -1079,14 +1086,14 @@ struct Batch {
   int launch_resample() {
     const RsGroup& gs = rs_groups[0];
     if (gs.m) HIPCHECK(pa_launch_step(3, 0, 0, 0, 0, gs.blocks, 0, stream, gs.desc_d, gs.offs_d, gs.m));
-    for (int wi = 0; wi < 2; ++wi) {
+    for (int wi = 0; wi < kWidths; ++wi) {
       const RsGroup& g = rs_groups[1 + wi];
       if (g.m)
-        HIPCHECK(pa_launch_step(10, PA_MSC, wi ? 64 : 32, 0, rs_nt[wi], g.blocks, rs_lds[wi], stream,
+        HIPCHECK(pa_launch_step(10, PA_MSC, width_of(wi), 0, rs_nt[wi], g.blocks, rs_lds[wi], stream,
                                 g.desc_d, g.offs_d, g.m));
-      const RsGroup& gw = rs_groups[3 + wi];
+      const RsGroup& gw = rs_groups[1 + kWidths + wi];
       if (gw.m)
-        HIPCHECK(pa_launch_step(17, PA_MSC, wi ? 64 : 32, 0, kWinJ, gw.blocks, kWinLds, stream,
+        HIPCHECK(pa_launch_step(17, PA_MSC, width_of(wi), 0, kWinJ, gw.blocks, kWinLds, stream,
                                 gw.desc_d, gw.offs_d, gw.m));
     }
     return PA_OK;
@@ -1721,7 +1728,7 @@ int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
         // sw = 32 may still fit with one sublattice in LDS (the plan decides)
         if ((size_t)p->L * p->L * (sw / 8) + 2048 > kLdsLimit && sw != 32) continue;
         cands.push_back({sw, nt, -1});
-        if (nt <= 512) cands.push_back({sw, nt, 1});
+        if (nt <= 512 && p->msbits >= 32) cands.push_back({sw, nt, 1});
       }
     }
   }

@@ -394,9 +394,10 @@ __device__ void msc_energy_slice(const S* lat, int L, int Ld2, int Nh, int* accE
   }
 }
 
-// ReplicaInit MSC:169-188 (w=32: words 2k, 2k+1; w=64: (word 4k << 32) + word 4k+1 and
-// (word 4k+2 << 32) + word 4k+3, first draw in the high half), written straight to HBM;
-// the energies follow from a zero-sweep pass of k_sweep_msc.
+// ReplicaInit MSC:169-188 (w<=32: words 2k, 2k+1, of which w = 8 / 16 keep the low byte /
+// half, MSC:173-178; w=64: (word 4k << 32) + word 4k+1 and (word 4k+2 << 32) + word 4k+3,
+// first draw in the high half), written straight to HBM; the energies follow from a
+// zero-sweep pass of k_sweep_msc.
 template <typename W>
 __global__ __launch_bounds__(128) void k_init_msc(const RunDesc* __restrict__ runs,
                                                   const int32_t* __restrict__ offs, int nruns,
@@ -411,7 +412,7 @@ __global__ __launch_bounds__(128) void k_init_msc(const RunDesc* __restrict__ ru
   const int Nh = d.Nh;
   W* gl = (W*)d.pop[0] + (size_t)B * d.N;
   const uint64_t base = is0[r] + (uint64_t)B * kStreamsPerUnit;
-  constexpr int per = (w == 32) ? 1 : 2;  // stream words per lattice word
+  constexpr int per = (w <= 32) ? 1 : 2;  // stream words per lattice word
   const int t = threadIdx.x;
   U4 blk{0, 0, 0, 0};
   uint64_t cached = ~0ull;
@@ -440,7 +441,9 @@ __global__ __launch_bounds__(128) void k_init_msc(const RunDesc* __restrict__ ru
 // Q partial sums: exp(-dB (E - Emean)) summed in the reference tree order.
 // MSC lane sums (QKer part 0, MSC:321-329): a block of w threads per group; here one lane
 // per thread, the warp-32 shuffle tree per 32 lanes, and for w = 64 the two warp sums
-// added as warp 0 of a 64-thread block does.
+// added as warp 0 of a 64-thread block does.  For w = 8 and 16 the reference reduces with
+// smallblockReduceSum (MSC:96-105): shared[t] += shared[t + stride], stride = w/2 .. 1,
+// which a shuffle-down tree of width w reproduces in lane 0 of each group.
 // ============================================================================================
 __global__ __launch_bounds__(1024) void k_qgroup(const RunDesc* __restrict__ runs,
                                                  const int32_t* __restrict__ offs, int nruns) {
@@ -458,10 +461,14 @@ __global__ __launch_bounds__(1024) void k_qgroup(const RunDesc* __restrict__ run
   double f = 0.0;
   if (g < rg && ((d.amask[st->buf][g] >> lane) & 1))
     f = exp_fp64(-dB * ((double)d.IE[n] - Emean));
-  f = warp32_tree(f);
-  if (w == 64) {
-    const double hi = __shfl_down(f, 32, 64);
-    f = f + hi;
+  if (w < 32) {
+    for (int o = w >> 1; o > 0; o >>= 1) f += __shfl_down(f, o, w);
+  } else {
+    f = warp32_tree(f);
+    if (w == 64) {
+      const double hi = __shfl_down(f, 32, 64);
+      f = f + hi;
+    }
   }
   if (lane == 0 && g < rg) put_dev(d.gsum + g, f);
   // QKer part 1 (MSC:330-335) in the run's last block: tree sums of 1024 group sums each
@@ -850,6 +857,17 @@ __global__ __launch_bounds__(NT) void k_resample_msc(const RunDesc* __restrict__
 // R_new), gather, write one 128-byte line per new group.  A window whose parents exceed
 // kWinSlots rows is processed in halves (a single new group needs at most w rows).  Run
 // blocks: windows (capacity / J) x site-chunk ranges (rs_split).
+// 32-bit word m (little-endian) of an array of words of type W.
+template <typename W, int NW>
+__device__ __forceinline__ uint32_t word32_of(const W (&a)[NW], int m) {
+  if constexpr (sizeof(W) == 8) return (m & 1) ? (uint32_t)(a[m >> 1] >> 32) : (uint32_t)a[m >> 1];
+  else if constexpr (sizeof(W) == 4) return a[m];
+  else if constexpr (sizeof(W) == 2) return (uint32_t)a[2 * m] | ((uint32_t)a[2 * m + 1] << 16);
+  else
+    return (uint32_t)a[4 * m] | ((uint32_t)a[4 * m + 1] << 8) | ((uint32_t)a[4 * m + 2] << 16) |
+           ((uint32_t)a[4 * m + 3] << 24);
+}
+
 template <typename W>
 __global__ __launch_bounds__(256) void k_resample_win(const RunDesc* __restrict__ runs,
                                                       const int32_t* __restrict__ offs, int nruns) {
@@ -960,13 +978,9 @@ __global__ __launch_bounds__(256) void k_resample_win(const RunDesc* __restrict_
         }
         uint4* out = (uint4*)(dst + (size_t)j * N + s0);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          if constexpr (sizeof(W) == 4)
-            out[q] = make_uint4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
-          else
-            out[q] = make_uint4((uint32_t)acc[2 * q], (uint32_t)(acc[2 * q] >> 32),
-                                (uint32_t)acc[2 * q + 1], (uint32_t)(acc[2 * q + 1] >> 32));
-        }
+        for (int q = 0; q < 8; ++q)
+          out[q] = make_uint4(word32_of(acc, 4 * q), word32_of(acc, 4 * q + 1),
+                              word32_of(acc, 4 * q + 2), word32_of(acc, 4 * q + 3));
       }
       __syncthreads();
     }
@@ -2060,7 +2074,12 @@ static hipError_t launch_sweep_msc(int w, int sw, int nt, int nblocks, size_t ld
     else if (nt == 256) PA_MSC(WT, ST, 256); \
     else PA_MSC(WT, ST, 128);               \
   } while (0)
-  if (w == 32) {
+  if (w == 8) {
+    PA_MSC_NT(uint8_t, uint8_t);
+  } else if (w == 16) {
+    if (sw == 16) PA_MSC_NT(uint16_t, uint16_t);
+    else PA_MSC_NT(uint16_t, uint8_t);
+  } else if (w == 32) {
     if (sw == 32) PA_MSC_NT(uint32_t, uint32_t);
     else if (sw == 16) PA_MSC_NT(uint32_t, uint16_t);
     else PA_MSC_NT(uint32_t, uint8_t);
@@ -2086,6 +2105,7 @@ static hipError_t launch_sweep_msc_bal(int w, int sw, int nt, int P, size_t lds,
     else if (nt == 256) PA_MSCB(WT, ST, 256);   \
     else PA_MSCB(WT, ST, 128);                  \
   } while (0)
+  if (w < 32) return hipErrorInvalidValue;  // no balanced instance for 8- and 16-bit words
   if (w == 32) {
     if (sw == 32) PA_MSCB_NT(uint32_t, uint32_t);
     else if (sw == 16) PA_MSCB_NT(uint32_t, uint16_t);
@@ -2188,6 +2208,23 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW_MSC(uint64_t, uint16_t);
   PA_ALLOW_MSC(uint64_t, uint8_t);
 #undef PA_ALLOW_MSC
+#define PA_ALLOW_MSC_NARROW(WT, ST)                   \
+  PA_ALLOW(k_sweep_msc<WT, ST, 128, false>);          \
+  PA_ALLOW(k_sweep_msc<WT, ST, 256, false>);          \
+  PA_ALLOW(k_sweep_msc<WT, ST, 512, false>);          \
+  PA_ALLOW(k_sweep_msc<WT, ST, 1024, false>);         \
+  PA_ALLOW(k_sweep_msc<WT, ST, 128, true>);           \
+  PA_ALLOW(k_sweep_msc<WT, ST, 256, true>);           \
+  PA_ALLOW(k_sweep_msc<WT, ST, 512, true>);           \
+  PA_ALLOW(k_sweep_msc<WT, ST, 1024, true>)
+  PA_ALLOW_MSC_NARROW(uint8_t, uint8_t);
+  PA_ALLOW_MSC_NARROW(uint16_t, uint16_t);
+  PA_ALLOW_MSC_NARROW(uint16_t, uint8_t);
+  PA_ALLOW((k_resample_msc<uint8_t, 256>));
+  PA_ALLOW((k_resample_msc<uint16_t, 256>));
+  PA_ALLOW((k_resample_win<uint8_t>));
+  PA_ALLOW((k_resample_win<uint16_t>));
+#undef PA_ALLOW_MSC_NARROW
 #undef PA_ALLOW
   return e;
 }
@@ -2201,13 +2238,18 @@ hipError_t pa_launch_init(int coding, int w, int sw, int nt, int half, int nbloc
     hipLaunchKernelGGL(k_init_ssc, dim3(nblocks), dim3(256), lds, s, runs, offs, nruns, is0);
     return hipGetLastError();
   }
-  if (w == 32)
+  if (w == 8)
+    hipLaunchKernelGGL(k_init_msc<uint8_t>, dim3(nblocks), dim3(128), 0, s, runs, offs, nruns, is0);
+  else if (w == 16)
+    hipLaunchKernelGGL(k_init_msc<uint16_t>, dim3(nblocks), dim3(128), 0, s, runs, offs, nruns, is0);
+  else if (w == 32)
     hipLaunchKernelGGL(k_init_msc<uint32_t>, dim3(nblocks), dim3(128), 0, s, runs, offs, nruns, is0);
   else
     hipLaunchKernelGGL(k_init_msc<uint64_t>, dim3(nblocks), dim3(128), 0, s, runs, offs, nruns, is0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (half) {
+    if (w < 32) return hipErrorInvalidValue;  // one sublattice in LDS: 32-bit slices only
     if (w == 32)
       hipLaunchKernelGGL((k_sweep_msc_half<uint32_t, true>), dim3(nblocks), dim3(1024), lds, s, runs, offs, nruns);
     else
@@ -2219,6 +2261,7 @@ hipError_t pa_launch_init(int coding, int w, int sw, int nt, int half, int nbloc
 
 hipError_t pa_occupancy(int kind, int coding, int w, int sw, int nt, size_t lds, int* blocks_per_cu) {
   if (kind != 11 && kind != 15) return hipErrorInvalidValue;
+  if (w < 32) return hipErrorInvalidValue;  // no balanced instance for 8- and 16-bit words
   (void)coding;
 #define PA_OCC(WT, ST, NTV)                                                                  \
   return kind == 15                                                                          \
@@ -2291,6 +2334,7 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 11: return launch_sweep_msc_bal<false>(w, sw, nt, nblocks, lds, s, runs, nruns);
     case 15: return launch_sweep_msc_bal<true>(w, sw, nt, nblocks, lds, s, runs, nruns);
     case 16:  // MSC sweeps with one sublattice in LDS (32-bit slices, 1024 threads)
+      if (w < 32) return hipErrorInvalidValue;
       if (w == 32)
         hipLaunchKernelGGL((k_sweep_msc_half<uint32_t, false>), dim3(nblocks), dim3(1024), lds, s, runs, offs, nruns);
       else
@@ -2300,7 +2344,11 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
     case 13: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 0); break;
     case 14: hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(1024), 0, s, runs, offs, nruns, 1); break;
     case 10:  // nt: threads per workgroup (32-bit words: 256 or 1024)
-      if (w == 32 && nt == 1024)
+      if (w == 8)
+        hipLaunchKernelGGL((k_resample_msc<uint8_t, 256>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+      else if (w == 16)
+        hipLaunchKernelGGL((k_resample_msc<uint16_t, 256>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
+      else if (w == 32 && nt == 1024)
         hipLaunchKernelGGL((k_resample_msc<uint32_t, 1024>), dim3(nblocks), dim3(1024), lds, s, runs, offs, nruns);
       else if (w == 32)
         hipLaunchKernelGGL((k_resample_msc<uint32_t, 256>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
@@ -2308,7 +2356,11 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
         hipLaunchKernelGGL((k_resample_msc<uint64_t, 256>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);
       break;
     case 17:  // MSC resampling by windows of new groups (whole 128-byte site chunks)
-      if (w == 32)
+      if (w == 8)
+        hipLaunchKernelGGL((k_resample_win<uint8_t>), dim3(nblocks), dim3(kWinJ), kWinLds, s, runs, offs, nruns);
+      else if (w == 16)
+        hipLaunchKernelGGL((k_resample_win<uint16_t>), dim3(nblocks), dim3(kWinJ), kWinLds, s, runs, offs, nruns);
+      else if (w == 32)
         hipLaunchKernelGGL((k_resample_win<uint32_t>), dim3(nblocks), dim3(kWinJ), kWinLds, s, runs, offs, nruns);
       else
         hipLaunchKernelGGL((k_resample_win<uint64_t>), dim3(nblocks), dim3(kWinJ), kWinLds, s, runs, offs, nruns);

@@ -35,7 +35,8 @@ def test_nbmax_exceeded(exe, tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("exe,coding,L,w", [("PAisingSSC", oracle.SSC, 16, 32), ("PAisingMSC", oracle.MSC, 16, 32),
-                                            ("PAisingMSC", oracle.MSC, 32, 64)])
+                                            ("PAisingMSC", oracle.MSC, 32, 64), ("PAisingMSC", oracle.MSC, 16, 8),
+                                            ("PAisingMSC", oracle.MSC, 32, 16)])
 def test_results_files_match_oracle(exe, coding, L, w, tmp_path):
     args = ["-R", 700, "-t", 5, "-d", 0.05, "-f", 0.6, "-s", 100, "-M", 2, "-L", L, "-P", 13]
     if coding == oracle.MSC:

@@ -79,6 +79,19 @@ def test_server_matches_oracle_and_plain_alone_and_in_mix(policy):
     srv.close()
 
 
+def test_batch_of_all_word_widths_matches_oracle():
+    """Runs of 8, 16, 32 and 64 replicas per word and single-spin runs in one batch."""
+    cases = [(32, pa.MSC, 8, 700, 4, 0.02, 1.0, 51, 5), (24, pa.MSC, 16, 900, 5, 0.02, 1.0, 52, 4),
+             (16, pa.MSC, 32, 1200, 3, 0.02, 1.0, 53, 6), (32, pa.MSC, 64, 600, 4, 0.02, 1.0, 54, 3),
+             (16, pa.SSC, 32, 400, 5, 0.02, 1.0, 55, 4), (48, pa.MSC, 8, 12000, 2, 0.05, 1.0, 56, 3)]
+    plist = [engine_params(*c) for c in cases]
+    b = pa.Batch(plist, use_graph=True)
+    b.run()
+    for i, c in enumerate(cases):
+        compare_rows(oracle_rows(*c), b.results(i))
+    b.close()
+
+
 def test_server_ticks_allocate_no_device_memory_once_reserved():
     """With the pool reserved up front, admitting and retiring a stream of requests makes no
     further device allocation, and every result stays bit-exact to the plain path."""

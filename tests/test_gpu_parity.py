@@ -22,6 +22,14 @@ CASES = [
     # populations whose parent chunks exceed the LDS budget: resampling by windows
     (16, pa.MSC, 32, 60000, 2, 0.05, 1.0, 13, 3),
     (16, pa.MSC, 64, 110000, 2, 0.05, 1.0, 17, 3),
+    # 8 and 16 replicas per word (PAisingMSC.cu:34 MSbits = 8, 16)
+    (16, pa.MSC, 8, 1000, 10, 0.005, 1.0, 100, 8),
+    (64, pa.MSC, 8, 2000, 10, 0.005, 1.0, 100, 5),
+    (16, pa.MSC, 16, 1000, 10, 0.005, 1.0, 100, 8),
+    (64, pa.MSC, 16, 2000, 10, 0.005, 1.0, 100, 5),
+    (24, pa.MSC, 8, 500, 4, 0.02, 1.0, 23, 5),
+    (96, pa.MSC, 16, 900, 3, 0.01, 1.0, 29, 4),
+    (16, pa.MSC, 8, 20000, 2, 0.05, 1.0, 13, 3),      # windowed resampling of 8-bit words
 ]
 
 
@@ -32,6 +40,16 @@ def test_engine_matches_oracle(case):
     orows = oracle_rows(L, coding, msbits, Rinit, eqs, dB, Bfin, seed, steps)
     erows, _ = pa.run_all(engine_params(L, coding, msbits, Rinit, eqs, dB, Bfin, seed, steps))
     assert compare_rows(orows, erows) == steps + 1
+
+
+@pytest.mark.parametrize("L,w,sw,nt", [(64, 16, 8, 256), (64, 8, 8, 128), (32, 16, 16, 1024),
+                                       (128, 8, 0, 512)])
+def test_narrow_word_block_shapes_match_oracle(L, w, sw, nt):
+    args = (L, pa.MSC, w, 600, 3, 0.01, 1.0, 31, 3)
+    print(f"case L={L} w={w} sw={sw} nt={nt}", flush=True)
+    orows = oracle_rows(*args)
+    erows, _ = pa.run_all(engine_params(*args, slice_bits=sw, threads=nt, balance=1))
+    assert compare_rows(orows, erows) == 4
 
 
 @pytest.mark.parametrize("uniform", [pa.UNIFORM_CUDA101, pa.UNIFORM_CUDA100])

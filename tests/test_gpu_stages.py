@@ -22,13 +22,16 @@ def params(L, coding, w=32, Rinit=1000, eqsweeps=3, seed=4242, uniform=0, **kw):
                           seed=seed, uniform=uniform, **kw)
 
 
+DT = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+
+
 def ssc_pop(L, R, rng):
     return rng.choice(np.array([-1, 1], np.int8), size=(R, L * L))
 
 
 def msc_pop(L, w, R, rng):
     rg = math.ceil(R / w)
-    dt = np.uint32 if w == 32 else np.uint64
+    dt = DT[w]
     return rng.integers(0, 2**w, size=(rg, L * L), dtype=np.uint64).astype(dt)
 
 
@@ -56,7 +59,10 @@ def test_ssc_sweep_stage(L, beta, uniform, nt):
                                                 (32, 64, 16, 256, -1, 0.2), (16, 32, 0, 0, 0, 0.0),
                                                 (80, 32, 0, 0, 0, 0.6), (128, 64, 8, 1024, -1, 0.44),
                                                 (64, 32, 32, 512, 1, 0.44), (48, 64, 0, 0, 0, 0.9),
-                                                (256, 64, 32, 0, 0, 0.44), (256, 32, 32, 0, 0, 0.0)])
+                                                (256, 64, 32, 0, 0, 0.44), (256, 32, 32, 0, 0, 0.0),
+                                                (64, 8, 0, 0, 0, 0.44), (16, 16, 0, 0, 0, 0.3),
+                                                (64, 16, 8, 256, -1, 0.6), (48, 8, 0, 0, 0, 1.0),
+                                                (96, 16, 0, 1024, 0, 0.44), (256, 16, 0, 0, 0, 0.44)])
 def test_msc_sweep_stage(L, w, sw, nt, bal, beta):
     rng = np.random.default_rng(L * w + sw)
     R, IS, sweeps = 5 * w - 7, 123456789, 2
@@ -75,7 +81,7 @@ def test_msc_sweep_stage(L, w, sw, nt, bal, beta):
 
 
 @pytest.mark.parametrize("coding,L,w", [(pa.SSC, 10, 32), (pa.SSC, 64, 32), (pa.MSC, 24, 32),
-                                        (pa.MSC, 64, 64)])
+                                        (pa.MSC, 64, 64), (pa.MSC, 32, 8), (pa.MSC, 48, 16)])
 def test_energy_stage(coding, L, w):
     rng = np.random.default_rng(L + w)
     R = 70
@@ -100,7 +106,9 @@ def test_energy_stage(coding, L, w):
                                                       (pa.SSC, 32, 3001, 2500, 1),
                                                       (pa.MSC, 32, 20000, 20000, 0),
                                                       (pa.MSC, 64, 5003, 5000, 1),
-                                                      (pa.MSC, 32, 1500, 2000, 0)])
+                                                      (pa.MSC, 32, 1500, 2000, 0),
+                                                      (pa.MSC, 8, 7001, 7000, 0),
+                                                      (pa.MSC, 16, 3000, 2900, 1)])
 def test_copy_counts_stage(coding, w, R, Rinit, uniform):
     """QKer + lnQ + CalcTauKer + CalcParSum on random energies (and random inactive lanes)."""
     rng = np.random.default_rng(R + w)
@@ -133,7 +141,8 @@ def test_copy_counts_stage(coding, w, R, Rinit, uniform):
 
 
 @pytest.mark.parametrize("coding,L,w", [(pa.SSC, 16, 32), (pa.SSC, 24, 32), (pa.MSC, 16, 32),
-                                        (pa.MSC, 64, 32), (pa.MSC, 32, 64), (pa.MSC, 24, 64)])
+                                        (pa.MSC, 64, 32), (pa.MSC, 32, 64), (pa.MSC, 24, 64),
+                                        (pa.MSC, 16, 8), (pa.MSC, 24, 16), (pa.MSC, 64, 8)])
 def test_resample_stage(coding, L, w):
     rng = np.random.default_rng(L * w + coding)
     R = 300
@@ -154,7 +163,7 @@ def test_resample_stage(coding, L, w):
         Rn_ref = int(Roff.sum())
         got, an, Rn = pa.stage_resample(p, pop, R, Roff, active)
         rgn = math.ceil(Rn_ref / w)
-        dt = np.uint32 if w == 32 else np.uint64
+        dt = DT[w]
         ref = np.zeros((rgn, L * L), dt)
         ran = np.zeros(rgn * w, np.uint8)
         IE = np.zeros(n, np.int32)
@@ -166,7 +175,9 @@ def test_resample_stage(coding, L, w):
 
 @pytest.mark.parametrize("L,w,R,pattern", [(16, 32, 60000, "uniform"), (16, 32, 60000, "shrink"),
                                            (24, 64, 100003, "uniform"), (16, 64, 100000, "shrink"),
-                                           (16, 64, 100000, "few"), (20, 32, 60000, "uniform")])
+                                           (16, 64, 100000, "few"), (20, 32, 60000, "uniform"),
+                                           (16, 8, 20000, "uniform"), (16, 8, 20000, "few"),
+                                           (16, 16, 30000, "shrink"), (24, 8, 15000, "uniform")])
 def test_resample_stage_large_populations(L, w, R, pattern):
     """Populations whose parent chunks exceed the LDS budget (resampling by windows of new
     groups where N is a multiple of the 128-byte chunk, L=20 with 16-byte chunks otherwise):
@@ -190,7 +201,7 @@ def test_resample_stage_large_populations(L, w, R, pattern):
     print(f"case MSC resample L={L} w={w} R={R} {pattern} Rn={Rn_ref}", flush=True)
     got, an, Rn = pa.stage_resample(p, pop, R, Roff, active)
     rgn = math.ceil(Rn_ref / w)
-    dt = np.uint32 if w == 32 else np.uint64
+    dt = DT[w]
     ref = np.zeros((rgn, L * L), dt)
     ran = np.zeros(rgn * w, np.uint8)
     IE = np.zeros(n, np.int32)
@@ -200,7 +211,8 @@ def test_resample_stage_large_populations(L, w, R, pattern):
     assert Rn == Rn_ref and (got == ref).all() and (an == ran).all()
 
 
-@pytest.mark.parametrize("coding,w,R", [(pa.SSC, 32, 5000), (pa.MSC, 32, 4001), (pa.MSC, 64, 3000)])
+@pytest.mark.parametrize("coding,w,R", [(pa.SSC, 32, 5000), (pa.MSC, 32, 4001), (pa.MSC, 64, 3000),
+                                        (pa.MSC, 8, 2003), (pa.MSC, 16, 3000)])
 def test_averages_stage(coding, w, R):
     rng = np.random.default_rng(R)
     L = 64

@@ -33,6 +33,13 @@ def from_2d(g):
     return sub
 
 
+DTYPES = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+
+
+def random_words(rng, n, w):
+    return rng.integers(0, 2**w, size=n, dtype=np.uint64).astype(DTYPES[w])
+
+
 def energy_2d(g):
     return int(-(g * np.roll(g, -1, 1)).sum() - (g * np.roll(g, -1, 0)).sum()), int(g.sum())
 
@@ -53,14 +60,13 @@ def test_ssc_energy_brute_force(L):
     assert all(e % 4 == 0 and abs(e) <= 2 * L * L for e in IE)
 
 
-@pytest.mark.parametrize("w", [32, 64])
+@pytest.mark.parametrize("w", [8, 16, 32, 64])
 @pytest.mark.parametrize("L", [4, 8, 24])
 def test_msc_energy_equals_per_lane_ssc(w, L):
     rng = np.random.default_rng(w + L)
     rg = 3
     N = L * L
-    dt = np.uint32 if w == 32 else np.uint64
-    words = rng.integers(0, 2**w, size=rg * N, dtype=np.uint64).astype(dt)
+    words = random_words(rng, rg * N, w)
     IE = np.zeros(rg * w, np.int32)
     M = np.zeros(rg * w, np.int32)
     lib.ora_msc_energy(L, w, rg, P(words), P(IE), P(M))
@@ -114,7 +120,7 @@ def test_ssc_sweep_geometry_and_streams_2d(L):
         assert (sp[r] == ref[r]).all()
 
 
-@pytest.mark.parametrize("L,w", [(4, 32), (6, 64), (10, 32)])
+@pytest.mark.parametrize("L,w", [(4, 32), (6, 64), (10, 32), (4, 8), (6, 16), (10, 8)])
 def test_msc_sweep_per_lane_2d(L, w):
     """Each replica bit follows the MSC rule: its random is LCG step `bit` from the site's
     stream word; count>=2 antiparallel flips, count 1 iff ran<T4, count 0 iff ran<T8."""
@@ -122,9 +128,8 @@ def test_msc_sweep_per_lane_2d(L, w):
     N, Nh = L * L, L * L // 2
     T4 = math.ceil(4294967296.0 * math.exp(-4 * beta))
     T8 = math.ceil(4294967296.0 * math.exp(-8 * beta))
-    dt = np.uint32 if w == 32 else np.uint64
     rng = np.random.default_rng(L * w)
-    words = rng.integers(0, 2**w, size=rg * N, dtype=np.uint64).astype(dt)
+    words = random_words(rng, rg * N, w)
     lanes = [to_2d((2 * ((words.astype(np.uint64) >> np.uint64(b)) & np.uint64(1)).astype(np.int64) - 1), L)
              for b in range(w)]
     for s in range(1):
@@ -183,27 +188,63 @@ def test_resample_order_ssc():
     assert [int(x) for x in (out[:, 0])] == [sp[0, 0], sp[0, 0], sp[2, 0], sp[3, 0], sp[3, 0], sp[3, 0]]
 
 
-def test_resample_order_msc_transposed():
-    """Child n = prefix + p -> group n mod rg_new, lane n div rg_new (MSC:413-414)."""
-    L, w, rg_old = 4, 32, 2
+@pytest.mark.parametrize("w", [8, 16, 32, 64])
+def test_resample_order_msc_transposed(w):
+    """Child n = prefix + p -> group n mod rg_new, lane n div rg_new (MSC:413-414); for 8 and
+    16 bits the reference's atomicXor on the 32-bit word holding 4 / 2 sites sets bit
+    it_k + 8 (t & 3) / it_k + 16 (t & 1) of it (MSC:415-422), i.e. bit it_k of site t."""
+    L, rg_old = 4, 2
     N = 16
-    rng = np.random.default_rng(9)
-    words = rng.integers(0, 2**32, size=rg_old * N, dtype=np.uint64).astype(np.uint32)
+    rng = np.random.default_rng(9 + w)
+    words = random_words(rng, rg_old * N, w)
     act = np.ones(rg_old * w, np.uint8)
     IE = np.arange(rg_old * w, dtype=np.int32)
     Roff = rng.integers(0, 3, size=rg_old * w).astype(np.uint32)
     Rn = int(Roff.sum())
     rg_new = math.ceil(Rn / w)
-    wn = np.zeros(rg_new * N, np.uint32)
+    wn = np.zeros(rg_new * N, DTYPES[w])
     an = np.zeros(rg_new * w, np.uint8)
     IEn = np.zeros(rg_new * w, np.int32)
     lib.ora_msc_resample(L, w, rg_old, P(words), P(act), P(IE), P(Roff), Rn, P(wn), P(an), P(IEn))
     parents = np.repeat(np.arange(rg_old * w), Roff)
     for n in range(Rn):
         j, k = n % rg_new, n // rg_new
-        p = parents[n]
+        p = int(parents[n])
         pg, pb = divmod(p, w)
         for s in range(N):
             assert (int(wn[j * N + s]) >> k) & 1 == (int(words[pg * N + s]) >> pb) & 1
         assert an[j * w + k] == 1 and IEn[j * w + k] == IE[p]
     assert an.sum() == Rn
+
+
+@pytest.mark.parametrize("w", [8, 16])
+def test_msc_init_narrow_words_are_low_bits_of_one_stream_word(w):
+    """MSbits 8 / 16: gA[idx] = curand() & 0xFF / 0xFFFF, one stream word per site word in the
+    same order as 32 bits (MSC:173-178), so the words are the low bits of the 32-bit init's."""
+    L, rg, R, seed, IS = 6, 3, 3 * w - 2, 41, 1000
+    N = L * L
+    ref = np.zeros(rg * N, np.uint32)
+    act32 = np.zeros(rg * 32, np.uint8)
+    lib.ora_msc_init(L, 32, rg, R, seed, IS, P(ref), P(act32))
+    got = np.zeros(rg * N, DTYPES[w])
+    act = np.zeros(rg * w, np.uint8)
+    lib.ora_msc_init(L, w, rg, R, seed, IS, P(got), P(act))
+    assert (got.astype(np.uint32) == (ref & np.uint32(2**w - 1))).all()
+    assert act.sum() == R and (act.reshape(rg, w)[:-1] == 1).all()
+
+
+@pytest.mark.parametrize("w", [8, 16])
+def test_small_block_tree(w):
+    """smallblockReduceSum (MSC:96-105) pairs shared[t] with shared[t + stride] for stride =
+    w/2 .. 1; for 8 lanes that is ((v0+v4)+(v2+v6))+((v1+v5)+(v3+v7)).  It equals the
+    32-lane shuffle-down tree of the 32/64-bit path on the same values padded with zeros
+    (adding +0.0 is exact), which is how the device evaluates it."""
+    rng = np.random.default_rng(w)
+    for _ in range(200):
+        v = np.exp(rng.normal(0, 30, size=w))
+        got = lib.ora_small_block_reduce(P(v), w)
+        if w == 8:
+            assert got == ((v[0] + v[4]) + (v[2] + v[6])) + ((v[1] + v[5]) + (v[3] + v[7]))
+        pad = np.zeros(32)
+        pad[:w] = v
+        assert got == lib.ora_block_reduce(P(pad), 32)
