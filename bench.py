@@ -48,9 +48,14 @@ N = L * L
 #   MSC, per w-bit word: a quarter Philox4x32-10 block (10 rounds x {2 v_mad_u64_u32,
 #     2 v_bitop3} / 4 = 10), the bit-sliced neighbour count and acceptance (15 bitwise ops)
 #     and two bit reversals (27 ops);
-#   SSC, per site: a quarter Philox block (10), neighbour sum (2), spin product (1), table
-#     index (1), compare (1), select and negate (2): 17 ops.
-MSC_OPS_PER_BIT, MSC_OPS_PER_WORD, SSC_OPS_PER_SITE = 5, 27, 17
+#   SSC, per site, in the packed-byte form of the sweep (four sites of a row per dword): a
+#     quarter Philox block (10); per site a byte extract of the threshold index, a compare
+#     and an add-with-carry into the flip mask (3); per dword of four sites the horizontal
+#     byte permute, the count-and-index adds (3 + a shift), the flip-mask spread (1) and the
+#     masked xor (1) = 7, i.e. 1.75 per site: 14.75 ops.  (The per-site form counted 17:
+#     neighbour sum 3, product, table index, compare, select and negate.)
+MSC_OPS_PER_BIT, MSC_OPS_PER_WORD, SSC_OPS_PER_SITE = 5, 27, 14.75
+SSC_OPS_PER_SITE_SITEWISE = 17  # the other SSC paths (a variant with a forced block shape)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
@@ -466,13 +471,14 @@ def main():
     }
     if not args.no_ssc:
         s_ = timed_run(pa.SSC, args.steps, args.warmup, tune=not args.no_tune)
-        s_ach = s_["ev_attempts"] * SSC_OPS_PER_SITE / (s_["sweep_ms"] * 1e-3)
+        ssc_ops = SSC_OPS_PER_SITE if s_["variant"]["threads"] == 0 else SSC_OPS_PER_SITE_SITEWISE
+        s_ach = s_["ev_attempts"] * ssc_ops / (s_["sweep_ms"] * 1e-3)
         out["ssc"] = {"value": round(s_["attempts"] / (s_["wall"] * 1e9), 4),
                       "unit": "spin-flip attempts/ns",
                       "workload": "plain PA, SSC int8, L=64, R=20000, EqSw=100, dB=0.005, seed 100",
                       "ms_per_step": round(s_["wall"] * 1e3 / args.steps, 4),
                       "sweep_attempts_per_ns": round(s_["ev_attempts"] / (s_["sweep_ms"] * 1e6), 2),
-                      "valu_frac": round(s_ach / peak, 4),
+                      "valu_frac": round(s_ach / peak, 4), "ops_per_attempt": ssc_ops,
                       "resample_GBps": round(s_["rs_bytes"] / (s_["resample_ms"] * 1e-3) / 1e9, 1),
                       "variant": s_["variant"]}
     if not args.no_batch:

@@ -554,6 +554,24 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     else if (!aligned && ntmax <= 4) h.path = 2;
     else if (!aligned && ntmax <= 9) h.path = 3;
     else if (!aligned) h.path = 5;
+    // packed-byte sweeps (k_sweep_ssc_q) for aligned lattices from L = 64 to 256 unless a
+    // block shape is forced: one task per (four streams, Philox block), T = 32 nb tasks per
+    // half sweep, two per thread, the workgroup the power of two >= T / 2 (sweep kernel alone,
+    // R = 20000, attempts/ns, against the previous paths: L = 64 2151-2332 vs 1931, 96 1879-1940
+    // vs 1829, 128 2480 vs 1974, 160 2014 vs 1019, 224 1927, 256 2004 vs 1712).  L = 192 keeps
+    // its period-3 path (2006 vs 1831: 1152 tasks leave 44 % of the 1024-thread slots idle),
+    // L = 32 its own (one task per stream quad leaves most of a wave idle).  A replica then
+    // has only T / 128 waves, so a small population takes the other paths until its waves
+    // reach eight per CU (L = 64 R = 1000: 984 packed against 1479; R = 20000: 2233 vs 1909).
+    if (aligned && p.L >= 64 && p.L <= 256 && !(period3 && nb > 16) && !p.threads) {
+      const int T = 32 * nb;
+      int nt = 64;
+      while (nt < 1024 && 2 * nt < T) nt *= 2;
+      if ((int64_t)h.cap_units * (nt / 64) >= 8 * (int64_t)device_cus()) {
+        h.path = 7;
+        h.nt = nt;
+      }
+    }
   }
   if (p.coding == PA_MSC) {
     // 32-bit slices with one sublattice in LDS and the other in registers (msc_unit_half):
@@ -1306,8 +1324,9 @@ struct Batch {
 // Error bits of a run's state once its step records and state are on the host.
 int check_fetched(RunHost& h) {
   if (h.st_h.err)
-    return fail(PA_ERR_HIP, "balanced sweep hand-over wait timed out (device error bits " +
-                                std::to_string(h.st_h.err) + ")");
+    return fail(PA_ERR_HIP, "device error bits " + std::to_string(h.st_h.err) +
+                                " (1: a balanced-sweep hand-over wait timed out; 2: a sweep launch "
+                                "shape left tasks unswept)");
   if (h.st_h.overflow)
     return fail(PA_ERR_CAPACITY, "population outgrew its buffers");
   h.results_valid = true;
@@ -1722,6 +1741,7 @@ int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
   for (int nt : nts) {
     if (p->coding == PA_SSC && (nt == 1024 || (!aligned && nt != 128 && Nh <= 9 * 128))) continue;
     if (p->coding == PA_SSC) {
+      if (nt == nts[0]) cands.push_back({0, 0, 0});  // the plan's own shape (packed-byte sweeps)
       cands.push_back({0, nt, 0});
     } else {
       for (int sw = p->msbits; sw >= 8; sw /= 2) {

@@ -1366,6 +1366,181 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
   for (int i = threadIdx.x; i < N; i += NT) g[i] = lat8[i];
 }
 
+// ---- SSC sweeps on packed site bytes (aligned lattices, L <= 256) ----------------------
+// The streams of a replica are dealt to threads four at a time: a task is (quad q: streams
+// 4q..4q+3, Philox block m), and for each word jj of the block the four sites
+// 4q + j + 128 (4m + jj), j = 0..3, are four consecutive sites of one row (L/2 and 4q + 128 k
+// are multiples of 4), i.e. one aligned dword of the sublattice in LDS.  Spins are held in
+// LDS as bytes 0 (down) / 1 (up), so the four neighbours of all four sites are four dwords
+// (same index, above, below, and the horizontal neighbours taken from the adjacent group by
+// one byte permute), the up-neighbour counts c are four bytes of one add, and the threshold
+// index c + 5 s (s the own spin: 10 classes, threshold by the reference's product
+// (2s-1)(2c-4)) four bytes of another.  Per site: a byte extract, one LDS threshold read, a
+// compare and an add-with-carry that shifts the decision into a 4-bit mask, which one
+// multiply spreads back over the four bytes to flip them.
+//
+// Thresholds are compared as v <= V - 1 (v < V for V >= 1; V = 2^32, every move accepted,
+// included).  When a threshold is 0 (beta above about 2.8) the 64-bit form v < V is used.
+// Each thread takes MB = 2 tasks (tid and tid + NT): two tasks' eight Philox blocks in flight
+// ran 10-30 % faster than one task per thread at equal or higher lane occupancy (L = 96..256),
+// four tasks per thread spilled and ran at half speed.
+
+// Bit j of the result = [v_j <= T_j], j = 0..3: compares into lane masks, and an add with
+// carry-in per mask (b = 2 b + carry), ordered so that every mask is written four VALU
+// instructions before its carry-in read.
+__device__ __forceinline__ uint32_t ssc_accept4(uint32_t T0, uint32_t v0, uint32_t T1, uint32_t v1,
+                                                uint32_t T2, uint32_t v2, uint32_t T3, uint32_t v3) {
+  uint64_t m0, m1, m2, m3, cc;
+  uint32_t b;
+  asm("v_cmp_ge_u32_e64 %[m3], %[t3], %[v3]\n\t"
+      "v_cmp_ge_u32_e64 %[m2], %[t2], %[v2]\n\t"
+      "v_cmp_ge_u32_e64 %[m1], %[t1], %[v1]\n\t"
+      "v_cmp_ge_u32_e64 %[m0], %[t0], %[v0]\n\t"
+      "v_addc_co_u32_e64 %[b], %[cc], 0, 0, %[m3]\n\t"
+      "v_addc_co_u32_e64 %[b], %[cc], %[b], %[b], %[m2]\n\t"
+      "v_addc_co_u32_e64 %[b], %[cc], %[b], %[b], %[m1]\n\t"
+      "v_addc_co_u32_e64 %[b], %[cc], %[b], %[b], %[m0]"
+      : [b] "=&v"(b), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3),
+        [cc] "=&s"(cc)
+      : [t0] "v"(T0), [v0] "v"(v0), [t1] "v"(T1), [v1] "v"(v1), [t2] "v"(T2), [v2] "v"(v2),
+        [t3] "v"(T3), [v3] "v"(v3));
+  return b;
+}
+
+// Byte selectors of v_perm_b32(Dx, D0, sel) for the horizontal neighbours of a group
+// [x, x+3]: the group after (bytes D0[1..3], Dx[0]) or the group before (Dx[3], D0[0..2]).
+constexpr uint32_t kSelNext = 0x04030201u, kSelPrev = 0x02010007u;
+
+template <int NTMAX, int MB>
+__global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict__ runs,
+                                                      const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKUnits * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int B = blockIdx.x - off[r];
+  if (B >= st->Rnew) return;
+  extern __shared__ __align__(16) uint32_t latq[];  // N bytes: sublattice A then B
+  const int NT = blockDim.x;
+  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
+  const int step = st->step;
+  uint32_t* g = (uint32_t*)((int8_t*)d.pop[st->buf ^ 1] + (size_t)B * N);
+  // +1 -> 1, -1 -> 0: bit 1 of a byte is the sign
+  for (int i = threadIdx.x; i < N / 4; i += NT) latq[i] = (~g[i] >> 1) & 0x01010101u;
+  const uint64_t V1 = d.thr[3 * step + 0], V4 = d.thr[3 * step + 1], V8 = d.thr[3 * step + 2];
+  __shared__ uint32_t tabn[10];
+  __shared__ uint64_t tabw[10];
+  if (threadIdx.x < 10) {
+    const int i = (int)threadIdx.x, up = i >= 5, c = up ? i - 5 : i;
+    const int prod = (2 * up - 1) * (2 * c - 4);
+    const uint64_t V = prod == 4 ? V8 : prod == 2 ? V4 : V1;
+    tabw[i] = V;
+    tabn[i] = (uint32_t)(V - 1);
+  }
+  const bool wide = V4 == 0 || V8 == 0;
+  const uint64_t seed = d.seed;
+  const uint64_t sub0 = st->IS + (uint64_t)st->R + (uint64_t)B * kStreamsPerUnit;
+  // the thread's tasks and their fixed geometry (byte offsets within a sublattice)
+  const int nb = Nh >> 9, T = 32 * nb;
+  if (T > MB * NT) {  // a launch shape that leaves tasks unswept: refuse it (host error bit 2)
+    if (threadIdx.x == 0) atomicOr(&st->err, 2);
+    return;
+  }
+  int q[MB], m[MB], o0[MB], up0[MB], dn3[MB];
+  int hx[MB][2][4];
+  uint32_t sel[MB][2][4];
+#pragma unroll
+  for (int i = 0; i < MB; ++i) {
+    const int task = (int)threadIdx.x + i * NT;
+    q[i] = task & 31;
+    m[i] = task >> 5;
+    o0[i] = 4 * q[i] + 512 * m[i];
+    up0[i] = o0[i] < Ld2 ? o0[i] - Ld2 + Nh : o0[i] - Ld2;
+    dn3[i] = o0[i] + 384 + Ld2 >= Nh ? o0[i] + 384 + Ld2 - Nh : o0[i] + 384 + Ld2;
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const int o = o0[i] + 128 * jj;
+      const int ty = o / Ld2, tx = o - ty * Ld2;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // A (h = 0): odd rows neighbour tx + 1, even rows tx - 1; B the other way round
+        const bool next = ((ty & 1) != 0) == (h == 0);
+        hx[i][h][jj] = next ? (tx + 4 < Ld2 ? o + 4 : ty * Ld2) : (tx > 0 ? o - 4 : ty * Ld2 + Ld2 - 4);
+        sel[i][h][jj] = next ? kSelNext : kSelPrev;
+      }
+    }
+  }
+  __syncthreads();
+  const uint8_t* tb = (const uint8_t*)tabn;
+  for (uint32_t s = 0; s < d.eqsweeps; ++s) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint8_t* own8 = (uint8_t*)latq + h * Nh;
+      const uint8_t* oth8 = (const uint8_t*)latq + (h ^ 1) * Nh;
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        if ((int)threadIdx.x + i * NT >= T) continue;
+        const uint64_t blk = (uint64_t)(2 * s + h) * nb + m[i];
+        const U4 rb0 = stream_block(seed, sub0 + 4 * q[i] + 0, blk);
+        const U4 rb1 = stream_block(seed, sub0 + 4 * q[i] + 1, blk);
+        const U4 rb2 = stream_block(seed, sub0 + 4 * q[i] + 2, blk);
+        const U4 rb3 = stream_block(seed, sub0 + 4 * q[i] + 3, blk);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int o = o0[i] + 128 * jj;
+          const uint32_t S = *(const uint32_t*)(own8 + o);
+          const uint32_t D0 = *(const uint32_t*)(oth8 + o);
+          const uint32_t Dx = *(const uint32_t*)(oth8 + hx[i][h][jj]);
+          const uint32_t Du = *(const uint32_t*)(oth8 + (jj == 0 ? up0[i] : o - Ld2));
+          const uint32_t Dd = *(const uint32_t*)(oth8 + (jj == 3 ? dn3[i] : o + Ld2));
+          const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, sel[i][h][jj]);
+          // threshold index x 4 per byte: (c + 5 s) << 2 <= 36
+          const uint32_t t4 = (D0 + Dh + Du + Dd + (S << 2) + S) << 2;
+          const uint32_t v0 = u4_word(rb0, jj), v1 = u4_word(rb1, jj), v2 = u4_word(rb2, jj),
+                         v3 = u4_word(rb3, jj);
+          uint32_t bits;
+          if (!wide) {
+            bits = ssc_accept4(*(const uint32_t*)(tb + (t4 & 0xFFu)), v0,
+                               *(const uint32_t*)(tb + ((t4 >> 8) & 0xFFu)), v1,
+                               *(const uint32_t*)(tb + ((t4 >> 16) & 0xFFu)), v2,
+                               *(const uint32_t*)(tb + (t4 >> 24)), v3);
+          } else {
+            bits = ((uint64_t)v0 < tabw[(t4 & 0xFFu) >> 2] ? 1u : 0u) |
+                   ((uint64_t)v1 < tabw[((t4 >> 8) & 0xFFu) >> 2] ? 2u : 0u) |
+                   ((uint64_t)v2 < tabw[((t4 >> 16) & 0xFFu) >> 2] ? 4u : 0u) |
+                   ((uint64_t)v3 < tabw[(t4 >> 24) >> 2] ? 8u : 0u);
+          }
+          // bits b3..b0 -> bytes: b x (1 + 2^7 + 2^14 + 2^21) puts bit j at 8 j, no carries
+          *(uint32_t*)(own8 + o) = S ^ (__umul24(bits, 0x204081u) & 0x01010101u);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // back to +1 / -1 bytes for the energies (energyKer SSC:195-215) and the store
+  for (int i = threadIdx.x; i < N / 4; i += NT) latq[i] = ~(latq[i] * 0xFEu);
+  __syncthreads();
+  const int8_t* lat8 = (const int8_t*)latq;
+  int e = 0, mm = 0;
+  for (int idx = threadIdx.x; idx < Nh; idx += NT) {
+    const int ty = idx / Ld2, tx = idx - ty * Ld2;
+    const int iL = ty * Ld2 + (tx == 0 ? Ld2 - 1 : tx - 1);
+    const int iU = (ty == L - 1 ? 0 : ty + 1) * Ld2 + tx;
+    const int sA = lat8[idx], sB = lat8[Nh + idx];
+    int Ai2, Bi2;
+    if (ty & 1) { Ai2 = sB; Bi2 = lat8[iL]; } else { Ai2 = lat8[Nh + iL]; Bi2 = sA; }
+    const int Ai4 = lat8[Nh + iU], Bi4 = lat8[iU];
+    e += -sA * (Ai2 + Ai4) - sB * (Bi2 + Bi4);
+    mm += sA + sB;
+  }
+  __shared__ int shi[16];
+  const int E = block_sum_int<int>(e, shi);
+  const int Mg = block_sum_int<int>(mm, shi);
+  if (threadIdx.x == 0) { d.IE[B] = E; d.M[B] = Mg; }
+  for (int i = threadIdx.x; i < N / 4; i += NT) g[i] = latq[i];
+}
+
 // MSC sweeps of one unit = (group g, slice qs): sweeps [s_begin, s_end) of the step, with the
 // lattice slice staged in LDS.  A slice holds SW = 8*sizeof(S) consecutive replica bits of
 // every site word; its LCG chain starts at ran_{q*SW} = A_{q SW} ran_0 + C_{q SW}.  Slices
@@ -2185,6 +2360,7 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW((k_sweep_ssc<512, 6>));
   PA_ALLOW((k_sweep_ssc<256, 5>));
   PA_ALLOW((k_sweep_ssc<512, 5>));
+  PA_ALLOW((k_sweep_ssc_q<1024, 2>));
 #define PA_ALLOW_MSC(WT, ST)                          \
   PA_ALLOW((k_sweep_msc_bal<WT, ST, 128, false>));    \
   PA_ALLOW((k_sweep_msc_bal<WT, ST, 256, false>));    \
@@ -2311,6 +2487,10 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
             else if (nt == 128) hipLaunchKernelGGL((k_sweep_ssc<128, 5>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
             else return hipErrorInvalidValue;
           }
+        } else if (sw == 7) {
+          // packed-byte sweeps, two tasks per thread; nt is a whole number of waves
+          if (nt < 64 || nt > 1024 || nt % 64) return hipErrorInvalidValue;
+          hipLaunchKernelGGL((k_sweep_ssc_q<1024, 2>), dim3(nblocks), dim3(nt), lds, s, runs, offs, nruns);
         } else if (sw == 6) {
           if (nt == 512) hipLaunchKernelGGL((k_sweep_ssc<512, 6>), dim3(nblocks), dim3(512), lds, s, runs, offs, nruns);
           else if (nt == 256) hipLaunchKernelGGL((k_sweep_ssc<256, 6>), dim3(nblocks), dim3(256), lds, s, runs, offs, nruns);

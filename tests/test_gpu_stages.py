@@ -38,7 +38,12 @@ def msc_pop(L, w, R, rng):
 @pytest.mark.parametrize("L,beta,uniform,nt", [(16, 0.44, 0, 0), (24, 1.0, 1, 0), (64, 0.0, 0, 0),
                                                (64, 0.3, 1, 256), (96, 0.7, 0, 0), (96, 0.44, 1, 128),
                                                (192, 0.44, 0, 0), (48, 0.5, 0, 0),
-                                               (144, 0.44, 0, 0)])
+                                               (144, 0.44, 0, 0),
+                                               # packed-byte sweeps (default plan, L % 32 == 0)
+                                               (32, 0.44, 0, 0), (128, 0.44, 1, 0), (160, 0.7, 0, 0),
+                                               (224, 0.3, 1, 0), (256, 0.44, 0, 0), (64, 1.0, 1, 0),
+                                               # thresholds of 0 (exp(-8 beta) 2^32 < 1): wide compares
+                                               (64, 3.0, 0, 0), (96, 6.0, 1, 0), (256, 6.0, 0, 0)])
 def test_ssc_sweep_stage(L, beta, uniform, nt):
     rng = np.random.default_rng(L + nt)
     R, IS, sweeps = 37, 9876543210, 3
