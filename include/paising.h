@@ -57,8 +57,10 @@ typedef struct pa_params {
   int32_t uniform;           /* pa_uniform                                                */
   int32_t max_steps;         /* 0 = whole schedule; otherwise stop after this many steps  */
   int32_t trace;             /* 1 = record per-step population hashes (parity checks)     */
-  int32_t slice_bits;        /* MSC sweep slice width 8/16/32/64, 0 = autotuned           */
-  int32_t threads;           /* sweep workgroup size 128/256/512, 0 = autotuned           */
+  int32_t slice_bits;        /* MSC sweep slice width 8/16/32/64, 0 = autotuned; SSC:     */
+                             /* 1 = site-per-thread kernels instead of packed-byte ones   */
+  int32_t threads;           /* sweep workgroup size 128/256/512 (MSC also 1024; packed   */
+                             /* SSC any multiple of 64 up to 1024), 0 = autotuned         */
   int32_t balance;           /* MSC makespan-balanced sweeps: 0 auto, 1 on, -1 off        */
   int32_t adaptive;          /* 1 = adaptive beta step, the reference's AdaptiveStep build  */
                              /* (PAisingSSC.cu:23,644-660): plain runs only               */

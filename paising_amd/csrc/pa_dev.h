@@ -65,6 +65,8 @@ struct RunDesc {
   int32_t q128, r128;       // 128 / Ld2, 128 % Ld2 (site-walk increments)
   uint32_t ld2_magic;       // floor(2^32 / Ld2) + 1: idx / Ld2 == mulhi(idx, magic) for idx < 2^17
   int32_t rs_lp;            // MSC resampling: site chunks per 128-byte line (same XCD)
+  int32_t ssc_p;            // packed SSC sweeps (k_sweep_ssc_p): parts per stream quad
+  int32_t pad0;
   uint64_t seed;
   void* pop[2];             // spin buffers: SSC int8 [cap][N], MSC word [cap][N]
   uint64_t* amask[2];       // MSC active-lane bit masks per group (ping-pong with pop)
@@ -98,7 +100,8 @@ enum KernelKind : int32_t {
   kKGroupChunks = 2,  // q partials: 1024 replicas (SSC) or 1024 groups (MSC)
   kKAvg = 3,          // averages: 1024 replicas (SSC) or (1024 groups, lane) (MSC)
   kKResample = 4,     // resampling: one block per child replica (SSC) / site chunk (MSC)
-  kKindCount = 5
+  kKSweep = 5,        // packed SSC sweeps: one block per RPW replicas (k_sweep_ssc_p)
+  kKindCount = 6
 };
 
 }  // namespace pa

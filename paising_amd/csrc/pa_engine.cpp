@@ -363,6 +363,7 @@ struct RunHost {
   int w = 1, N = 0, n_steps = 0, cap_pop = 0, cap_units = 0;
   int R0 = 0;
   int sw = 0, nt = 0, path = 0;  // MSC path 1: one sublattice in LDS (msc_unit_half)
+  int ssc_p = 1, rpw = 1;  // SSC path 8: parts per stream quad, replicas per workgroup
   bool half = false;
   int rs_sites = 1;
   int rs_split = 1;
@@ -389,6 +390,7 @@ struct RunHost {
   int blocks(int kind) const {
     switch (kind) {
       case kKUnits: return cap_units;
+      case kKSweep: return ceil_div(cap_units, rpw);
       case kKChunks: return ceil_div(cap_pop, kRedBlock);
       case kKGroupChunks: return ceil_div(cap_units, kRedBlock);
       case kKAvg: return p.coding == kSSC ? ceil_div(cap_pop, kRedBlock)
@@ -401,6 +403,11 @@ struct RunHost {
                                 : ceil_div(ceil_div(N, rs_sites), 8 * rs_lp) * 8 * rs_lp * rs_split;
     }
     return 0;
+  }
+  // LDS of a packed SSC sweep workgroup (path 8): RPW lattices, and for one part per stream
+  // quad the geometry table (16 bytes per quad and dword)
+  size_t sweep_lds() const {
+    return (size_t)rpw * N + (ssc_p == 1 ? (size_t)32 * 16 * ceil_div(N / 2, 128) : 0);
   }
   size_t resample_lds() const {
     // parents' chunks plus one zero chunk
@@ -530,19 +537,50 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
   // half sweep are dealt to P threads
   const int nblk_u = ceil_div(ceil_div(Nh, 128), 4) + 1;
   const int P_u = Nh <= 512 ? 1 : nblk_u >= 8 ? 4 : nblk_u >= 4 ? 2 : 1;
+  // packed-byte SSC sweeps of any lattice with L % 8 == 0 (k_sweep_ssc_p, path 8) unless
+  // the aligned packed kernel takes it (decided below) or slice_bits = 1 asks for the
+  // site-per-thread kernels: one part per stream quad (the whole stream walk, RPW = NT / 32
+  // replicas per workgroup, no workgroup barrier) up to 12 dwords per quad and half sweep;
+  // beyond that P parts deal the Philox blocks of a half sweep, P the power of two nearest
+  // to a third of the blocks (measured at R = 20000 against P = 2 .. 32: L = 56 P = 2, 112
+  // P = 4, 144 P = 8, 208 and 240 P = 16 were the fastest).  Its word counter of a step must
+  // stay within 32 bits.
+  const int nq8 = ceil_div(Nh, 128);
+  int P8 = 1;
+  if (nq8 > 12) {
+    const double third = (ceil_div(nq8, 4) + 1) / 3.0;
+    P8 = 2;
+    while (P8 < 32 && third > P8 * 1.41421356) P8 *= 2;
+  }
+  const bool packed8 = p.coding == PA_SSC && p.L % 8 == 0 && p.slice_bits != 1 &&
+                       (!aligned || p.L < 64 || p.L > 256) &&
+                       2.0 * p.eqsweeps * nq8 < 4294967295.0;
   h.nt = p.threads ? p.threads : 128 * (aligned ? std::min(4, std::max(1, nb)) : P_u);
-  // 1024-thread workgroups (MSC only): twice the waves per CU for lattices whose slice
-  // leaves room for one workgroup per CU
-  if (h.nt != 128 && h.nt != 256 && h.nt != 512 && !(h.nt == 1024 && p.coding == PA_MSC))
+  if (packed8) {
+    // a given workgroup size caps the parts at threads / 32
+    const int nt8 = p.threads ? p.threads : P8 == 1 ? 256 : 64 * ceil_div(32 * P8, 64);
+    if (nt8 % 64 || nt8 < 64 || nt8 > 1024)
+      return fail(PA_ERR_INVALID, "threads must be a multiple of 64 from 64 to 1024 for "
+                                  "packed-byte SSC sweeps");
+    h.ssc_p = std::min(P8, nt8 / 32);
+    h.rpw = nt8 / (32 * h.ssc_p);
+    if (h.sweep_lds() + 1024 > kLdsLimit)
+      return fail(PA_ERR_INVALID, "packed-byte SSC workgroup exceeds LDS");
+    h.nt = nt8;
+    h.path = 8;
+  } else if (h.nt != 128 && h.nt != 256 && h.nt != 512 && !(h.nt == 1024 && p.coding == PA_MSC)) {
+    // 1024-thread workgroups (MSC only): twice the waves per CU for lattices whose slice
+    // leaves room for one workgroup per CU
     return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512 (or 1024 for MSC)");
-  if (!aligned && p.coding == PA_SSC && ceil_div(Nh, 128) <= 9) h.nt = 128;
+  }
+  if (!aligned && p.coding == PA_SSC && ceil_div(Nh, 128) <= 9 && !packed8) h.nt = 128;
   // SSC sweep kernel path (pa_kernels.hip, ssc_sweeps): columns repeating every third site
   // of a stream (L = 96, 192) for more than one Philox block per thread and half sweep (6),
   // register-cached block loop for two or three blocks per thread (1), fixed-column block
   // loop for more blocks when L/2 divides 128 (4), register-cached stream walk for unaligned
   // lattices with at most 4 / 9 sites per stream and half sweep (2 / 3), blocks dealt to
   // several threads per stream beyond that (5)
-  if (p.coding == PA_SSC) {
+  if (p.coding == PA_SSC && !packed8) {
     const int ntmax = ceil_div(Nh, 128);
     const int Ld2 = p.L / 2;
     // (only while Ld2 < 128: a stream's first site is then its only one in row 0, and its
@@ -563,7 +601,8 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     // L = 32 its own (one task per stream quad leaves most of a wave idle).  A replica then
     // has only T / 128 waves, so a small population takes the other paths until its waves
     // reach eight per CU (L = 64 R = 1000: 984 packed against 1479; R = 20000: 2233 vs 1909).
-    if (aligned && p.L >= 64 && p.L <= 256 && !(period3 && nb > 16) && !p.threads) {
+    if (aligned && p.L >= 64 && p.L <= 256 && !(period3 && nb > 16) && !p.threads &&
+        p.slice_bits != 1) {
       const int T = 32 * nb;
       int nt = 64;
       while (nt < 1024 && 2 * nt < T) nt *= 2;
@@ -690,6 +729,7 @@ void desc_scalars(RunHost& h) {
   d.rs_lp = h.rs_lp;
   d.rs_split = h.rs_split;
   d.ld2_magic = (uint32_t)((1ull << 32) / (uint64_t)d.Ld2 + 1);
+  d.ssc_p = h.ssc_p;
 }
 
 constexpr int kWaitForMemory = -1;  // internal: the server pool has no room for now
@@ -767,6 +807,8 @@ struct Group {  // runs sharing one sweep kernel variant
   int32_t* offs_d = nullptr;
   int units_blocks = 0;  // total kKUnits blocks
   size_t lds = 0;
+  int sweep_blocks = 0;  // SSC path 8: total kKSweep blocks and their LDS (RPW lattices)
+  size_t sweep_lds = 0;
   int bal_workers = 0;   // > 0: makespan-balanced MSC sweep with this many workers
   size_t bal_lds = 0;
   int bal_kind = 11;     // 15: every member takes the register-cached path (fewer registers)
@@ -979,21 +1021,27 @@ struct Batch {
         std::vector<RunDesc> gd(m);
         std::vector<int32_t> go((size_t)kKindCount * (m + 1), 0);
         std::vector<uint64_t> gis(m);
-        int acc = 0;
-        size_t lds = 0;
+        int acc = 0, acc8 = 0;
+        size_t lds = 0, lds8 = 0;
         for (int i = 0; i < m; ++i) {
           const RunHost& h = *runs[g.members[i]];
           gd[i] = h.desc;
           gis[i] = h.p.initial_sequence;
           go[(size_t)kKUnits * (m + 1) + i] = acc;
+          go[(size_t)kKSweep * (m + 1) + i] = acc8;
           acc += h.cap_units;
+          acc8 += h.blocks(kKSweep);
           const size_t l = h.p.coding == PA_SSC ? (size_t)h.N
                            : (size_t)(h.half ? h.N / 2 : h.N) * (h.sw / 8);
           lds = std::max(lds, l);
+          lds8 = std::max(lds8, h.path == 8 ? h.sweep_lds() : 0);
         }
         go[(size_t)kKUnits * (m + 1) + m] = acc;
+        go[(size_t)kKSweep * (m + 1) + m] = acc8;
         g.units_blocks = acc;
         g.lds = lds;
+        g.sweep_blocks = acc8;
+        g.sweep_lds = lds8;
         if ((rc = alloc_table((void**)&g.desc_d, m * sizeof(RunDesc)))) return rc;
         if ((rc = alloc_table((void**)&g.offs_d, go.size() * 4))) return rc;
         HIPCHECK(put(g.desc_d, gd.data(), m * sizeof(RunDesc)));
@@ -1129,6 +1177,9 @@ struct Batch {
                               g.lds, s, g.desc_d, g.offs_d, (int)g.members.size()));
     else if (g.bal_workers > 0)
       HIPCHECK(pa_launch_step(g.bal_kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, s,
+                              g.desc_d, g.offs_d, (int)g.members.size()));
+    else if (g.v.coding == PA_SSC && g.v.path == 8)
+      HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, 8, g.v.nt, g.sweep_blocks, g.sweep_lds, s,
                               g.desc_d, g.offs_d, (int)g.members.size()));
     else
       HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.coding == PA_SSC ? g.v.path : g.v.sw, g.v.nt,
@@ -1735,21 +1786,23 @@ int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
   int rc = plan_run(*p, probe, 10.0);
   if (rc) return rc;
   std::vector<std::array<int, 3>> cands;
-  const int Nh = p->L * p->L / 2;
-  const bool aligned = (Nh % 512) == 0;
   const int nts[4] = {128, 256, 512, 1024};
+  if (p->coding == PA_SSC) {
+    // the plan's own shape, then workgroup sizes of the packed-byte kernels (where the
+    // lattice takes them) and of the site-per-thread kernels (slice_bits = 1); shapes a
+    // kernel does not take fail at creation and are skipped
+    cands.push_back({0, 0, 0});
+    for (int nt : {64, 128, 256, 512, 1024}) cands.push_back({0, nt, 0});
+    cands.push_back({1, 0, 0});
+    for (int nt : {128, 256, 512}) cands.push_back({1, nt, 0});
+  }
   for (int nt : nts) {
-    if (p->coding == PA_SSC && (nt == 1024 || (!aligned && nt != 128 && Nh <= 9 * 128))) continue;
-    if (p->coding == PA_SSC) {
-      if (nt == nts[0]) cands.push_back({0, 0, 0});  // the plan's own shape (packed-byte sweeps)
-      cands.push_back({0, nt, 0});
-    } else {
-      for (int sw = p->msbits; sw >= 8; sw /= 2) {
-        // sw = 32 may still fit with one sublattice in LDS (the plan decides)
-        if ((size_t)p->L * p->L * (sw / 8) + 2048 > kLdsLimit && sw != 32) continue;
-        cands.push_back({sw, nt, -1});
-        if (nt <= 512 && p->msbits >= 32) cands.push_back({sw, nt, 1});
-      }
+    if (p->coding == PA_SSC) break;
+    for (int sw = p->msbits; sw >= 8; sw /= 2) {
+      // sw = 32 may still fit with one sublattice in LDS (the plan decides)
+      if ((size_t)p->L * p->L * (sw / 8) + 2048 > kLdsLimit && sw != 32) continue;
+      cands.push_back({sw, nt, -1});
+      if (nt <= 512 && p->msbits >= 32) cands.push_back({sw, nt, 1});
     }
   }
   double best = 1e300;
@@ -2027,7 +2080,7 @@ namespace {
 
 struct NoAllocTables {
   RunDesc desc;
-  int32_t offs[kKindCount * 2];  // launch table of one run (only the per-unit row is read)
+  int32_t offs[kKindCount * 2];  // launch table of one run (the per-unit and packed-sweep rows)
   RunState st;
   uint64_t thr[6];               // thresholds of step 1 at [3..5]
   double beta[2];
@@ -2110,6 +2163,8 @@ extern "C" int pa_stage_sweep_noalloc(const pa_params* p, int R, uint64_t is, do
   t.desc.dbeta = dev->dbeta;
   t.offs[kKUnits * 2 + 0] = 0;
   t.offs[kKUnits * 2 + 1] = units;
+  t.offs[kKSweep * 2 + 0] = 0;
+  t.offs[kKSweep * 2 + 1] = h.blocks(kKSweep);
   // step 1 of a run whose sweep streams start at `is` (no parents: R = 0), reading the
   // population from buffer buf ^ 1 = 0, as pa_stage_sweep sets it up
   t.st.step = 1;
@@ -2128,6 +2183,9 @@ extern "C" int pa_stage_sweep_noalloc(const pa_params* p, int R, uint64_t is, do
   if (h.p.coding == PA_MSC && h.half)
     HIPCHECK(pa_launch_step(16, PA_MSC, h.w, 32, 1024, units * (h.w / 32), lds, s, &dev->desc,
                             dev->offs, 1));
+  else if (h.p.coding == PA_SSC && h.path == 8)
+    HIPCHECK(pa_launch_step(4, PA_SSC, 1, 8, h.nt, h.blocks(kKSweep), h.sweep_lds(), s,
+                            &dev->desc, dev->offs, 1));
   else
     HIPCHECK(pa_launch_step(4, h.p.coding, h.w, h.p.coding == PA_SSC ? h.path : h.sw, h.nt,
                             units * (h.p.coding == PA_SSC ? 1 : h.w / h.sw), lds, s, &dev->desc,

@@ -1541,6 +1541,237 @@ __global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict
   for (int i = threadIdx.x; i < N / 4; i += NT) g[i] = latq[i];
 }
 
+// Packed-byte SSC sweeps of any lattice with L % 8 == 0 (aligned or not, small or large).
+// Stream quad q (streams 4q .. 4q+3 of a replica) owns the dwords at byte offsets
+// 4q + 128 k, k < n_q = ceil((N/2 - 4q) / 128), of each sublattice: the k-th sites of its four
+// streams side by side in one row (L/2 % 4 == 0 keeps a dword inside a row, and N/2 % 4 == 0
+// gives the four streams the same n_q).  In half sweep j = 2s + h the quad's k-th dword takes
+// word j n_q + k of each of its four streams, so Philox block b of the four streams serves the
+// dwords k = 4b + i - j n_q (i < 4) that fall in [0, n_q).  The blocks of a half sweep are
+// dealt to P parts of the quad by b mod P; a block that spans two half sweeps stays with its
+// part and is evaluated once (the last block is kept in registers).  A workgroup holds
+// RPW = NT / (32 P) replicas of one run, each in its own N-byte LDS slot (several small
+// lattices per workgroup, or one large lattice over 32 P lanes).  The four dwords of a block
+// are straight-line code: a word outside [0, n_q) reads dword 0 and stores to a dummy LDS
+// word, so their LDS reads overlap.  Site geometry (horizontal neighbour dword and byte
+// selector, rows above and below) comes from a per-workgroup table of (k, q) entries when
+// P = 1 (at most 12 dwords per quad), else from the dword's offset.  Dword update,
+// thresholds and acceptance as in k_sweep_ssc_q.
+constexpr uint32_t kSelSum = kSelNext + kSelPrev;
+
+// Geometry of dword o of a sublattice in half h: byte offsets of the horizontal neighbour
+// dword, the dwords above and below (other sublattice), and the byte selector.
+__device__ __forceinline__ void ssc_p_geo(int o, int h, int L, int Ld2, int Nh, uint32_t magic,
+                                          int& hx, int& up, int& dn, uint32_t& sel) {
+  const int ty = (int)mulhi32((uint32_t)o, magic);
+  const int tx = o - (int)__umul24((uint32_t)ty, (uint32_t)Ld2);
+  // A (h = 0): odd rows neighbour tx + 1, even rows tx - 1; B the other way round
+  const bool next = ((ty ^ h) & 1) != 0;
+  const int hxn = o + 4 - (tx + 4 == Ld2 ? Ld2 : 0);
+  const int hxp = o - 4 + (tx == 0 ? Ld2 : 0);
+  hx = next ? hxn : hxp;
+  sel = next ? kSelNext : kSelPrev;
+  up = o - Ld2 + (ty == 0 ? Nh : 0);
+  dn = o + Ld2 - (ty == L - 1 ? Nh : 0);
+}
+
+template <bool WIDE, bool TAB>
+__device__ __forceinline__ void ssc_p_sweeps(uint8_t* lat8, const uint4* geo, uint32_t* dummy,
+                                             const RunDesc& d, uint32_t nsw, uint64_t seed,
+                                             uint64_t sub, int q, int part, int P, int nq,
+                                             bool walk, const uint8_t* tb, const uint64_t* tabw) {
+  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh;
+  const uint32_t magic = d.ld2_magic;
+  U4 c0{0, 0, 0, 0}, c1{0, 0, 0, 0}, c2{0, 0, 0, 0}, c3{0, 0, 0, 0};
+  uint32_t cb = 0xFFFFFFFFu;   // block held in c0..c3
+  uint32_t b = (uint32_t)part;  // smallest block of this part not yet passed
+  for (uint32_t s = 0; s < nsw; ++s) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint8_t* own8 = lat8 + h * Nh;
+      const uint8_t* oth8 = lat8 + (h ^ 1) * Nh;
+      if (walk) {
+        const uint32_t w0 = (2 * s + h) * (uint32_t)nq;
+        const uint32_t bA = w0 >> 2, bB = (w0 + (uint32_t)nq - 1) >> 2;
+        // the previous half sweep's last block may continue into this one
+        if (b >= (uint32_t)P && b - (uint32_t)P >= bA) b -= (uint32_t)P;
+        for (; b <= bB; b += (uint32_t)P) {
+          if (b != cb) {
+            c0 = stream_block(seed, sub + 0, b);
+            c1 = stream_block(seed, sub + 1, b);
+            c2 = stream_block(seed, sub + 2, b);
+            c3 = stream_block(seed, sub + 3, b);
+            cb = b;
+          }
+          const int k0 = (int)(4 * b - w0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int k = k0 + i;
+            const bool ok = k >= 0 && k < nq;
+            // one part: a few dwords per half sweep, skip the words outside it
+            if (TAB && !ok) continue;
+            const int kk = ok ? k : 0;
+            int o, hx, up, dn;
+            uint32_t sel;
+            if constexpr (TAB) {
+              const uint4 e = geo[kk * 32 + q];
+              o = (int)e.w;
+              hx = h ? (int)(e.x >> 16) : (int)(e.x & 0xFFFFu);
+              up = (int)(e.y & 0xFFFFu);
+              dn = (int)(e.y >> 16);
+              sel = h ? kSelSum - e.z : e.z;
+            } else {
+              o = 4 * q + 128 * kk;
+              ssc_p_geo(o, h, L, Ld2, Nh, magic, hx, up, dn, sel);
+            }
+            const uint32_t S = *(const uint32_t*)(own8 + o);
+            const uint32_t D0 = *(const uint32_t*)(oth8 + o);
+            const uint32_t Dx = *(const uint32_t*)(oth8 + hx);
+            const uint32_t Du = *(const uint32_t*)(oth8 + up);
+            const uint32_t Dd = *(const uint32_t*)(oth8 + dn);
+            const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, sel);
+            // threshold index x 4 per byte: (c + 5 s) << 2 <= 36
+            const uint32_t t4 = (D0 + Dh + Du + Dd + (S << 2) + S) << 2;
+            const uint32_t v0 = u4_word(c0, i), v1 = u4_word(c1, i), v2 = u4_word(c2, i),
+                           v3 = u4_word(c3, i);
+            uint32_t bits;
+            if constexpr (!WIDE) {
+              bits = ssc_accept4(*(const uint32_t*)(tb + (t4 & 0xFFu)), v0,
+                                 *(const uint32_t*)(tb + ((t4 >> 8) & 0xFFu)), v1,
+                                 *(const uint32_t*)(tb + ((t4 >> 16) & 0xFFu)), v2,
+                                 *(const uint32_t*)(tb + (t4 >> 24)), v3);
+            } else {
+              bits = ((uint64_t)v0 < tabw[(t4 & 0xFFu) >> 2] ? 1u : 0u) |
+                     ((uint64_t)v1 < tabw[((t4 >> 8) & 0xFFu) >> 2] ? 2u : 0u) |
+                     ((uint64_t)v2 < tabw[((t4 >> 16) & 0xFFu) >> 2] ? 4u : 0u) |
+                     ((uint64_t)v3 < tabw[(t4 >> 24) >> 2] ? 8u : 0u);
+            }
+            uint32_t* dst = ok ? (uint32_t*)(own8 + o) : dummy;
+            *dst = S ^ (__umul24(bits, 0x204081u) & 0x01010101u);
+          }
+        }
+      }
+      // one part: a replica's 32 lanes lie in one wavefront, whose LDS operations complete
+      // in issue order, so the half sweeps need only a fence against compiler reordering
+      if constexpr (TAB) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      } else {
+        __syncthreads();
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__ offs, int nruns) {
+  const int* off = offs + kKSweep * (nruns + 1);
+  const int r = find_run(off, nruns, blockIdx.x);
+  const RunDesc& d = runs[r];
+  RunState* st = d.st;
+  if (!st->active || st->overflow) return;
+  const int NT = blockDim.x;
+  const int P = d.ssc_p, lanes = 32 * P, RPW = NT / lanes;
+  const int Rn = st->Rnew;
+  const int B0 = (blockIdx.x - off[r]) * RPW;
+  if (RPW < 1 || RPW > 32) {  // a launch shape the plan never makes: refuse (host error bit 2)
+    if (threadIdx.x == 0) atomicOr(&st->err, 2);
+    return;
+  }
+  if (B0 >= Rn) return;
+  // RPW slots of N bytes (A then B), then for P = 1 the geometry table: 32 n_max entries
+  extern __shared__ __align__(16) uint32_t latq[];
+  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
+  const int slot = threadIdx.x / lanes;  // threads beyond RPW * lanes only join the barriers
+  const int lr = threadIdx.x - slot * lanes;
+  const int B = B0 + slot;
+  const bool live = slot < RPW && B < Rn;
+  const int q = lr & 31, part = lr >> 5;
+  uint8_t* lat8 = (uint8_t*)latq + (size_t)(live ? slot : 0) * N;
+  uint32_t* g = (uint32_t*)((int8_t*)d.pop[st->buf ^ 1] + (size_t)(live ? B : 0) * N);
+  // +1 -> 1, -1 -> 0: bit 1 of a byte is the sign
+  if (live)
+    for (int i = lr; i < N / 4; i += lanes) ((uint32_t*)lat8)[i] = (~g[i] >> 1) & 0x01010101u;
+  const bool tab = P == 1;
+  uint4* geo = (uint4*)((uint8_t*)latq + (size_t)RPW * N);
+  if (tab) {
+    const int nmax = (Nh + 127) >> 7;
+    for (int i = threadIdx.x; i < 32 * nmax; i += NT) {
+      const int qq = i & 31, k = i >> 5, o = 4 * qq + 128 * k;
+      if (o < Nh) {
+        int hx0, hx1, up, dn;
+        uint32_t sel0, sel1;
+        ssc_p_geo(o, 0, L, Ld2, Nh, d.ld2_magic, hx0, up, dn, sel0);
+        ssc_p_geo(o, 1, L, Ld2, Nh, d.ld2_magic, hx1, up, dn, sel1);
+        geo[i] = make_uint4((uint32_t)hx0 | ((uint32_t)hx1 << 16),
+                            (uint32_t)up | ((uint32_t)dn << 16), sel0, (uint32_t)o);
+      }
+    }
+  }
+  const int step = st->step;
+  const uint64_t V1 = d.thr[3 * step + 0], V4 = d.thr[3 * step + 1], V8 = d.thr[3 * step + 2];
+  __shared__ uint32_t tabn[10];
+  __shared__ uint64_t tabw[10];
+  __shared__ int accE[32], accM[32];
+  __shared__ uint32_t dummy;
+  if (threadIdx.x < 10) {
+    const int i = (int)threadIdx.x, up = i >= 5, c = up ? i - 5 : i;
+    const int prod = (2 * up - 1) * (2 * c - 4);
+    const uint64_t V = prod == 4 ? V8 : prod == 2 ? V4 : V1;
+    tabw[i] = V;
+    tabn[i] = (uint32_t)(V - 1);
+  }
+  if (threadIdx.x < 32) { accE[threadIdx.x] = 0; accM[threadIdx.x] = 0; }
+  const bool wide = __builtin_amdgcn_readfirstlane((int)(V4 == 0 || V8 == 0)) != 0;
+  const uint64_t sub = st->IS + (uint64_t)st->R + (uint64_t)B * kStreamsPerUnit + 4 * q;
+  const int nq = (Nh - 4 * q + 127) >> 7;
+  const bool walk = live && nq > 0;
+  __syncthreads();
+  const uint8_t* tb = (const uint8_t*)tabn;
+  if (wide) {
+    if (tab) ssc_p_sweeps<true, true>(lat8, geo, &dummy, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
+    else ssc_p_sweeps<true, false>(lat8, geo, &dummy, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
+  } else {
+    if (tab) ssc_p_sweeps<false, true>(lat8, geo, &dummy, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
+    else ssc_p_sweeps<false, false>(lat8, geo, &dummy, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
+  }
+  // back to +1 / -1 bytes for the energies (energyKer SSC:195-215) and the store
+  if (live)
+    for (int i = lr; i < N / 4; i += lanes) ((uint32_t*)lat8)[i] = ~(((uint32_t*)lat8)[i] * 0xFEu);
+  __syncthreads();
+  int e = 0, mm = 0;
+  if (live) {
+    const int8_t* l8 = (const int8_t*)lat8;
+    for (int idx = lr; idx < Nh; idx += lanes) {
+      const int ty = idx / Ld2, tx = idx - ty * Ld2;
+      const int iL = ty * Ld2 + (tx == 0 ? Ld2 - 1 : tx - 1);
+      const int iU = (ty == L - 1 ? 0 : ty + 1) * Ld2 + tx;
+      const int sA = l8[idx], sB = l8[Nh + idx];
+      int Ai2, Bi2;
+      if (ty & 1) { Ai2 = sB; Bi2 = l8[iL]; } else { Ai2 = l8[Nh + iL]; Bi2 = sA; }
+      const int Ai4 = l8[Nh + iU], Bi4 = l8[iU];
+      e += -sA * (Ai2 + Ai4) - sB * (Bi2 + Bi4);
+      mm += sA + sB;
+    }
+  }
+  // integer sums: exact in any order (32-lane groups, then LDS atomics per slot)
+#pragma unroll
+  for (int m = 16; m >= 1; m >>= 1) {
+    e += __shfl_xor(e, m, 32);
+    mm += __shfl_xor(mm, m, 32);
+  }
+  if (live && q == 0) {
+    atomicAdd(&accE[slot], e);
+    atomicAdd(&accM[slot], mm);
+  }
+  __syncthreads();
+  if (live) {
+    if (lr == 0) { d.IE[B] = accE[slot]; d.M[B] = accM[slot]; }
+    for (int i = lr; i < N / 4; i += lanes) g[i] = ((const uint32_t*)lat8)[i];
+  }
+}
+
 // MSC sweeps of one unit = (group g, slice qs): sweeps [s_begin, s_end) of the step, with the
 // lattice slice staged in LDS.  A slice holds SW = 8*sizeof(S) consecutive replica bits of
 // every site word; its LCG chain starts at ran_{q*SW} = A_{q SW} ran_0 + C_{q SW}.  Slices
@@ -2361,6 +2592,7 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW((k_sweep_ssc<256, 5>));
   PA_ALLOW((k_sweep_ssc<512, 5>));
   PA_ALLOW((k_sweep_ssc_q<1024, 2>));
+  PA_ALLOW(k_sweep_ssc_p);
 #define PA_ALLOW_MSC(WT, ST)                          \
   PA_ALLOW((k_sweep_msc_bal<WT, ST, 128, false>));    \
   PA_ALLOW((k_sweep_msc_bal<WT, ST, 256, false>));    \
@@ -2487,6 +2719,10 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
             else if (nt == 128) hipLaunchKernelGGL((k_sweep_ssc<128, 5>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
             else return hipErrorInvalidValue;
           }
+        } else if (sw == 8) {
+          // packed-byte sweeps of any L % 8 == 0: RPW = nt / (32 P) replicas per workgroup
+          if (nt < 64 || nt > 1024 || nt % 64) return hipErrorInvalidValue;
+          hipLaunchKernelGGL(k_sweep_ssc_p, dim3(nblocks), dim3(nt), lds, s, runs, offs, nruns);
         } else if (sw == 7) {
           // packed-byte sweeps, two tasks per thread; nt is a whole number of waves
           if (nt < 64 || nt > 1024 || nt % 64) return hipErrorInvalidValue;

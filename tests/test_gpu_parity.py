@@ -118,14 +118,20 @@ def test_ssc_block_shapes_match_oracle(nt):
     assert compare_rows(orows, erows) == 4
 
 
-@pytest.mark.parametrize("L,nt", [(96, 0), (96, 256), (96, 128), (192, 0), (192, 256), (48, 0), (16, 0), (10, 0), (6, 0), (4, 0),
-                                  (130, 0), (160, 0), (64, 128), (128, 0), (128, 128), (256, 0),
-                                  (80, 0), (144, 256), (112, 128), (384, 0), (384, 128)])
-def test_ssc_lattices_off_the_fast_path_match_oracle(L, nt):
+@pytest.mark.parametrize("L,nt,sb", [(96, 0, 0), (96, 256, 0), (96, 128, 0), (192, 0, 0), (192, 256, 0),
+                                     (48, 0, 0), (16, 0, 0), (10, 0, 0), (6, 0, 0), (4, 0, 0),
+                                     (130, 0, 0), (160, 0, 0), (64, 128, 0), (128, 0, 0), (128, 128, 0),
+                                     (256, 0, 0), (80, 0, 0), (144, 256, 0), (112, 128, 0), (384, 0, 0),
+                                     (384, 128, 0), (24, 64, 0), (40, 0, 0), (208, 0, 0),
+                                     # the site-per-thread kernels where the packed ones are the plan
+                                     (48, 0, 1), (16, 0, 1), (80, 0, 1), (144, 256, 1), (112, 128, 1),
+                                     (384, 128, 1)])
+def test_ssc_lattices_off_the_fast_path_match_oracle(L, nt, sb):
     """SSC lattices whose half sweeps do not split into whole Philox blocks per thread
-    (stream walk), or need several blocks per thread, at every block shape."""
+    (stream walk, packed-byte walk), or need several blocks per thread, at every block shape;
+    sb = 1 keeps the site-per-thread kernels."""
     args = (L, pa.SSC, 32, 120, 3, 0.02, 1.0, 29, 3)
-    print(f"case L={L} nt={nt}", flush=True)
+    print(f"case L={L} nt={nt} sb={sb}", flush=True)
     orows = oracle_rows(*args)
-    erows, _ = pa.run_all(engine_params(*args, threads=nt))
+    erows, _ = pa.run_all(engine_params(*args, threads=nt, slice_bits=sb))
     assert compare_rows(orows, erows) == 4

@@ -43,13 +43,42 @@ def msc_pop(L, w, R, rng):
                                                (32, 0.44, 0, 0), (128, 0.44, 1, 0), (160, 0.7, 0, 0),
                                                (224, 0.3, 1, 0), (256, 0.44, 0, 0), (64, 1.0, 1, 0),
                                                # thresholds of 0 (exp(-8 beta) 2^32 < 1): wide compares
-                                               (64, 3.0, 0, 0), (96, 6.0, 1, 0), (256, 6.0, 0, 0)])
+                                               (64, 3.0, 0, 0), (96, 6.0, 1, 0), (256, 6.0, 0, 0),
+                                               # packed-byte sweeps of any L % 8 == 0: several
+                                               # lattices per workgroup (one part per stream
+                                               # quad) or one lattice over several parts
+                                               (8, 0.44, 0, 0), (16, 0.7, 1, 64), (24, 0.44, 0, 1024),
+                                               (40, 0.3, 1, 128), (48, 0.44, 0, 512), (56, 0.6, 0, 0),
+                                               (80, 0.44, 1, 0), (112, 1.0, 0, 256), (144, 0.44, 0, 1024),
+                                               (208, 0.5, 1, 0), (240, 0.44, 0, 0), (48, 6.0, 0, 0),
+                                               (144, 6.0, 1, 0), (320, 0.44, 0, 0)])
 def test_ssc_sweep_stage(L, beta, uniform, nt):
     rng = np.random.default_rng(L + nt)
     R, IS, sweeps = 37, 9876543210, 3
     p = params(L, pa.SSC, eqsweeps=sweeps, uniform=uniform, threads=nt)
     pop = ssc_pop(L, R, rng)
     print(f"case SSC sweep L={L} beta={beta} uniform={uniform} nt={nt}", flush=True)
+    got, IE, M = pa.stage_sweep(p, pop, R, IS, beta)
+    ref = np.ascontiguousarray(pop.copy())
+    lib.ora_ssc_sweep(L, R, P(ref), sweeps, p.seed, IS, beta, UMAP[uniform])
+    assert (got == ref).all()
+    rIE = np.zeros(R, np.int32)
+    rM = np.zeros(R, np.int32)
+    lib.ora_ssc_energy(L, R, P(ref), P(rIE), P(rM))
+    assert (IE == rIE).all() and (M == rM).all()
+
+
+@pytest.mark.parametrize("L,beta,uniform,nt", [(16, 0.44, 0, 0), (24, 1.0, 1, 0), (48, 0.5, 0, 0),
+                                               (80, 0.44, 1, 0), (144, 0.44, 0, 256),
+                                               (112, 0.7, 0, 128), (64, 0.44, 0, 0)])
+def test_ssc_sweep_stage_site_kernels(L, beta, uniform, nt):
+    """slice_bits = 1 keeps SSC on the site-per-thread kernels (stream walks, block loops),
+    which must stay bit-exact where the packed-byte kernels now take the default plan."""
+    rng = np.random.default_rng(3 * L + nt)
+    R, IS, sweeps = 29, 1234567, 3
+    p = params(L, pa.SSC, eqsweeps=sweeps, uniform=uniform, threads=nt, slice_bits=1)
+    pop = ssc_pop(L, R, rng)
+    print(f"case SSC site-kernel sweep L={L} beta={beta} uniform={uniform} nt={nt}", flush=True)
     got, IE, M = pa.stage_sweep(p, pop, R, IS, beta)
     ref = np.ascontiguousarray(pop.copy())
     lib.ora_ssc_sweep(L, R, P(ref), sweeps, p.seed, IS, beta, UMAP[uniform])
