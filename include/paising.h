@@ -58,7 +58,9 @@ typedef struct pa_params {
   int32_t max_steps;         /* 0 = whole schedule; otherwise stop after this many steps  */
   int32_t trace;             /* 1 = record per-step population hashes (parity checks)     */
   int32_t slice_bits;        /* MSC sweep slice width 8/16/32/64, 0 = autotuned; SSC:     */
-                             /* 1 = site-per-thread kernels instead of packed-byte ones   */
+                             /* 1 = site-per-thread kernels instead of packed-byte ones,  */
+                             /* 2 = the general packed-byte kernel also where L/2 x L    */
+                             /* is a multiple of 1024 (L % 8 == 0 lattices)               */
   int32_t threads;           /* sweep workgroup size 128/256/512 (MSC also 1024; packed   */
                              /* SSC any multiple of 64 up to 1024), 0 = autotuned         */
   int32_t balance;           /* MSC makespan-balanced sweeps: 0 auto, 1 on, -1 off        */

@@ -553,7 +553,7 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     while (P8 < 32 && third > P8 * 1.41421356) P8 *= 2;
   }
   const bool packed8 = p.coding == PA_SSC && p.L % 8 == 0 && p.slice_bits != 1 &&
-                       (!aligned || p.L < 64 || p.L > 256) &&
+                       (!aligned || p.L < 64 || p.L > 256 || p.slice_bits == 2) &&
                        2.0 * p.eqsweeps * nq8 < 4294967295.0;
   h.nt = p.threads ? p.threads : 128 * (aligned ? std::min(4, std::max(1, nb)) : P_u);
   if (packed8) {
@@ -574,6 +574,10 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     return fail(PA_ERR_INVALID, "threads must be 128, 256 or 512 (or 1024 for MSC)");
   }
   if (!aligned && p.coding == PA_SSC && ceil_div(Nh, 128) <= 9 && !packed8) h.nt = 128;
+  // MSC lattices of at most 9 sites per stream and half sweep (L <= 48): 256-thread
+  // workgroups (R = 20000, attempts/ns against 128 threads: L = 16 2091 vs 1976, 24 1953
+  // vs 1860, 32 3370 vs 3159, 40 2347 vs 1915)
+  if (p.coding == PA_MSC && !p.threads && Nh <= 9 * 128 && h.nt < 256) h.nt = 256;
   // SSC sweep kernel path (pa_kernels.hip, ssc_sweeps): columns repeating every third site
   // of a stream (L = 96, 192) for more than one Philox block per thread and half sweep (6),
   // register-cached block loop for two or three blocks per thread (1), fixed-column block
@@ -847,6 +851,11 @@ void plan_balance(Group& g, const std::vector<RunHost*>& runs) {
     fast = fast && Nh % 512 == 0 && Nh / 512 <= g.v.nt / 128 && 128 % (h.p.L / 2) == 0;
   }
   if (force < 0) return;
+  // not by default for lattices of at most 4 sites per stream and half sweep (L <= 32):
+  // a balanced worker keeps one wave per SIMD busy, where whole units per workgroup fill
+  // more (R = 20000: L = 16 2091 vs 1611, 24 1953 vs 1489, 32 3370 vs 2577 attempts/ns)
+  for (int m : g.members)
+    if (force == 0 && runs[m]->N / 2 <= 4 * 128) return;
   const int cus = device_cus();
   const int64_t expect = units * 9 / 10;
   if (force == 0 && (expect < cus || expect >= 12 * (int64_t)cus)) return;

@@ -299,6 +299,45 @@ __device__ __forceinline__ void lcg_masks(uint32_t (&ran)[NW], uint32_t T4, uint
   }
 }
 
+// Condition masks of one word's SW LCG values as four chains of SW/4 steps each, started at
+// ran_0, ran_{SW/4}, ran_{SW/2}, ran_{3SW/4} by compile-time jumps (three multiply-adds), so
+// that a thread with a single word in flight still has four independent chains.
+constexpr uint32_t lcg_jump_a(int k) {
+  uint32_t a = 1u;
+  for (int i = 0; i < k; ++i) a *= kLcgA;
+  return a;
+}
+constexpr uint32_t lcg_jump_c(int k) {
+  uint32_t c = 0u;
+  for (int i = 0; i < k; ++i) c = kLcgA * c + kLcgC;
+  return c;
+}
+template <uint32_t A, uint32_t C>
+__device__ __forceinline__ uint32_t lcg_apply(uint32_t r) {
+  uint64_t p, carry;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(p), "=s"(carry) : "s"(A), "v"(r), "v"((uint64_t)C));
+  return (uint32_t)p;
+}
+template <typename S>
+__device__ __forceinline__ void lcg_masks_split(uint32_t r0, uint32_t T4, uint32_t T8, S& c4, S& c8) {
+  constexpr int Q = (int)sizeof(S) * 2;  // steps per chain: SW / 4
+  static_assert(Q >= 4 && Q <= 16, "16-, 32- or 64-bit slices");
+  uint32_t ch[4] = {r0, lcg_apply<lcg_jump_a(Q), lcg_jump_c(Q)>(r0),
+                    lcg_apply<lcg_jump_a(2 * Q), lcg_jump_c(2 * Q)>(r0),
+                    lcg_apply<lcg_jump_a(3 * Q), lcg_jump_c(3 * Q)>(r0)};
+  uint32_t a4[4] = {0u, 0u, 0u, 0u}, a8[4] = {0u, 0u, 0u, 0u};
+  const uint64_t c64 = kLcgC;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) lcg_cmp_step4(a4, a8, ch, T4, T8, c64);
+  c4 = 0;
+  c8 = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    c4 |= (S)((uint64_t)(__builtin_bitreverse32(a4[j]) >> (32 - Q)) << (Q * j));
+    c8 |= (S)((uint64_t)(__builtin_bitreverse32(a8[j]) >> (32 - Q)) << (Q * j));
+  }
+}
+
 // LCG jump: ran_j = A_j ran_0 + C_j (mod 2^32).
 __device__ __forceinline__ void lcg_jump(int j, uint32_t& Aj, uint32_t& Cj) {
   uint32_t a = 1u, c = 0u;
@@ -1988,7 +2027,10 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
           if ((p >> 2) != cached) { cached = p >> 2; rb = stream_block(seed, sub0 + t, cached); }
           uint32_t ran[1] = {JA * u4_word(rb, (int)(p & 3)) + JC};
           S c4[1], c8[1];
-          lcg_masks<S, 1>(ran, T4, T8, c4, c8);
+          // one word in flight: its LCG values as four jump-started chains (16-bit slices
+          // and wider)
+          if constexpr (SW >= 16) lcg_masks_split<S>(ran[0], T4, T8, c4[0], c8[0]);
+          else lcg_masks<S, 1>(ran, T4, T8, c4, c8);
           const Site q = site_at(t + kStreamsPerUnit * k, h, d);
           const S sp = own[q.idx];
           own[q.idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[q.idx], oth[q.i3], oth[q.i4], c4[0], c8[0]);

@@ -125,11 +125,14 @@ def test_ssc_block_shapes_match_oracle(nt):
                                      (384, 128, 0), (24, 64, 0), (40, 0, 0), (208, 0, 0),
                                      # the site-per-thread kernels where the packed ones are the plan
                                      (48, 0, 1), (16, 0, 1), (80, 0, 1), (144, 256, 1), (112, 128, 1),
-                                     (384, 128, 1)])
+                                     (384, 128, 1),
+                                     # the general packed kernel on lattices whose half sweeps
+                                     # split into whole Philox blocks (sb = 2)
+                                     (64, 0, 2), (96, 320, 2), (128, 64, 2), (256, 0, 2)])
 def test_ssc_lattices_off_the_fast_path_match_oracle(L, nt, sb):
     """SSC lattices whose half sweeps do not split into whole Philox blocks per thread
     (stream walk, packed-byte walk), or need several blocks per thread, at every block shape;
-    sb = 1 keeps the site-per-thread kernels."""
+    sb = 1 keeps the site-per-thread kernels, sb = 2 takes the general packed-byte kernel."""
     args = (L, pa.SSC, 32, 120, 3, 0.02, 1.0, 29, 3)
     print(f"case L={L} nt={nt} sb={sb}", flush=True)
     orows = oracle_rows(*args)
