@@ -463,7 +463,8 @@ int default_slice(int w, int N, int64_t groups, bool half_ok, bool* half) {
 }
 
 // Validate, size and compute the per-step tables of a run (no device work).  co_work: sweep
-// work of the other runs that share this run's launches (groups x sites), 0 for a lone run.
+// work of the other runs that share this run's launches (MSC groups x sites, single-spin
+// replicas x sites), 0 for a lone run.
 int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_work = 0) {
   pa_params p = p_in;
   {
@@ -552,8 +553,17 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     P8 = 2;
     while (P8 < 32 && third > P8 * 1.41421356) P8 *= 2;
   }
+  // One part gives a replica 32 lanes (a quarter of the site kernels' waves), so with at
+  // most 4 dwords per quad it needs enough replicas in the launch (this run's plus, in a
+  // batch, co_work / N of the other single-spin runs): from 5000 with up to 3 dwords, 16000
+  // with 4 (lone runs, sweep attempts/ns, packed against site kernels: L = 16 R = 4000 790
+  // vs 825, 6000 998 vs 886; L = 32 R = 8000 1661 vs 1769, 16000 1922 vs 1945, 20000 1933
+  // vs 1779; L = 40, 48 from R = 2000 on 695 vs 663 and 870 vs 802).
+  const int64_t reps8 = (int64_t)p.Rinit + co_work / h.N;
+  const bool few8 = P8 == 1 && ((nq8 <= 3 && reps8 < 5000) || (nq8 == 4 && reps8 < 16000));
   const bool packed8 = p.coding == PA_SSC && p.L % 8 == 0 && p.slice_bits != 1 &&
                        (!aligned || p.L < 64 || p.L > 256 || p.slice_bits == 2) &&
+                       (!few8 || p.slice_bits == 2) &&
                        2.0 * p.eqsweeps * nq8 < 4294967295.0;
   h.nt = p.threads ? p.threads : 128 * (aligned ? std::min(4, std::max(1, nb)) : P_u);
   if (packed8) {
@@ -1802,6 +1812,12 @@ int pa_autotune(const pa_params* p, pa_params* tuned, double* best_sweep_ms) {
     // kernel does not take fail at creation and are skipped
     cands.push_back({0, 0, 0});
     for (int nt : {64, 128, 256, 512, 1024}) cands.push_back({0, nt, 0});
+    const int Nh = p->L * p->L / 2;
+    if (p->L % 8 == 0 && !(Nh % 512 == 0 && p->L >= 64 && p->L <= 256)) {
+      // the packed kernel where the plan prefers the site kernels (small populations)
+      cands.push_back({2, 0, 0});
+      for (int nt : {64, 128, 256, 512}) cands.push_back({2, nt, 0});
+    }
     cands.push_back({1, 0, 0});
     for (int nt : {128, 256, 512}) cands.push_back({1, nt, 0});
   }
@@ -2208,7 +2224,8 @@ int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out
   auto b = std::make_unique<pa_batch>();
   int rc;
   if ((rc = b->batch.create_stream())) return rc;
-  // MSC runs of one word width share the sweep launches
+  // MSC runs of one word width share the sweep launches; single-spin runs count the other
+  // single-spin runs' replicas x sites
   auto msc_work = [&](int i) {
     return p[i].coding == PA_MSC && p[i].msbits > 0
                ? (int64_t)ceil_div(p[i].Rinit, p[i].msbits) * p[i].L * p[i].L : (int64_t)0;
@@ -2216,8 +2233,13 @@ int pa_batch_create(const pa_params* p, int nruns, int use_graph, pa_batch** out
   for (int i = 0; i < nruns; ++i) {
     if (p[i].adaptive) return fail(PA_ERR_INVALID, "adaptive runs take the plain path (pa_run_*)");
     int64_t co = 0;
-    for (int j = 0; j < nruns; ++j)
-      if (j != i && p[j].msbits == p[i].msbits) co += msc_work(j);
+    for (int j = 0; j < nruns; ++j) {
+      if (j == i) continue;
+      if (p[i].coding == PA_SSC)
+        co += p[j].coding == PA_SSC ? (int64_t)p[j].Rinit * p[j].L * p[j].L : 0;
+      else if (p[j].msbits == p[i].msbits)
+        co += msc_work(j);
+    }
     auto h = std::make_unique<RunHost>();
     if ((rc = plan_run(p[i], *h, 10.0, co))) return rc;
     if ((rc = materialize(*h))) return rc;

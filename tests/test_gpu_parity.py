@@ -122,7 +122,7 @@ def test_ssc_block_shapes_match_oracle(nt):
                                      (48, 0, 0), (16, 0, 0), (10, 0, 0), (6, 0, 0), (4, 0, 0),
                                      (130, 0, 0), (160, 0, 0), (64, 128, 0), (128, 0, 0), (128, 128, 0),
                                      (256, 0, 0), (80, 0, 0), (144, 256, 0), (112, 128, 0), (384, 0, 0),
-                                     (384, 128, 0), (24, 64, 0), (40, 0, 0), (208, 0, 0),
+                                     (384, 128, 0), (24, 64, 2), (16, 0, 2), (40, 0, 0), (208, 0, 0),
                                      # the site-per-thread kernels where the packed ones are the plan
                                      (48, 0, 1), (16, 0, 1), (80, 0, 1), (144, 256, 1), (112, 128, 1),
                                      (384, 128, 1),

@@ -46,16 +46,20 @@ def msc_pop(L, w, R, rng):
                                                (64, 3.0, 0, 0), (96, 6.0, 1, 0), (256, 6.0, 0, 0),
                                                # packed-byte sweeps of any L % 8 == 0: several
                                                # lattices per workgroup (one part per stream
-                                               # quad) or one lattice over several parts
-                                               (8, 0.44, 0, 0), (16, 0.7, 1, 64), (24, 0.44, 0, 1024),
-                                               (40, 0.3, 1, 128), (48, 0.44, 0, 512), (56, 0.6, 0, 0),
-                                               (80, 0.44, 1, 0), (112, 1.0, 0, 256), (144, 0.44, 0, 1024),
-                                               (208, 0.5, 1, 0), (240, 0.44, 0, 0), (48, 6.0, 0, 0),
-                                               (144, 6.0, 1, 0), (320, 0.44, 0, 0)])
+                                               # quad; nt < 0 asks for them with -nt threads
+                                               # where a small population would take the site
+                                               # kernels) or one lattice over several parts
+                                               (8, 0.44, 0, -256), (16, 0.7, 1, -64), (24, 0.44, 0, -1024),
+                                               (32, 0.6, 1, -128), (40, 0.3, 1, 128), (48, 0.44, 0, 512),
+                                               (56, 0.6, 0, 0), (80, 0.44, 1, 0), (112, 1.0, 0, 256),
+                                               (144, 0.44, 0, 1024), (208, 0.5, 1, 0), (240, 0.44, 0, 0),
+                                               (48, 6.0, 0, 0), (144, 6.0, 1, 0), (320, 0.44, 0, 0),
+                                               (16, 6.0, 0, -256)])
 def test_ssc_sweep_stage(L, beta, uniform, nt):
-    rng = np.random.default_rng(L + nt)
+    rng = np.random.default_rng(L + abs(nt))
     R, IS, sweeps = 37, 9876543210, 3
-    p = params(L, pa.SSC, eqsweeps=sweeps, uniform=uniform, threads=nt)
+    p = params(L, pa.SSC, eqsweeps=sweeps, uniform=uniform, threads=abs(nt),
+               slice_bits=2 if nt < 0 else 0)
     pop = ssc_pop(L, R, rng)
     print(f"case SSC sweep L={L} beta={beta} uniform={uniform} nt={nt}", flush=True)
     got, IE, M = pa.stage_sweep(p, pop, R, IS, beta)
