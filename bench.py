@@ -21,6 +21,9 @@ uneven shapes run as one batch against the plain path one run at a time, runs/s 
 bit-exactness) and "served" (Poisson arrivals at 20 requests/s: p50/p99 latency, runs/s and
 how far the offered work exceeds what the device completes; then the same request shapes at
 60 % of the measured capacity).  --mode batch / serve print those alone.
+
+--mode lattice --L 16,24,... --coding ssc,msc --R 1000,20000 --steps 3 --warmup 1: the sweep
+kernel's rate per lattice size (one JSON line per shape; default plan, or --tune).
 """
 import argparse
 import json
@@ -205,6 +208,45 @@ def bench_config3(args):
             "value": round(att / (wall * 1e9), 2), "unit": "spin-flip attempts/ns",
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 2),
             "R": [r.R for r in rows]}
+
+
+def lattice_rate(L, coding, R, eqsw, steps, warmup, tune):
+    """Sweep-kernel rate of one plain run (HIP events around the sweep launches of `steps`
+    beta steps after `warmup`): attempts/ns and the variant the plan (or the autotuner) chose."""
+    p = pa.make_params(L=L, coding=coding, msbits=32, Rinit=R, eqsweeps=eqsw, dB=DB, Bfin=1.0,
+                       seed=SEED)
+    if tune:
+        p, _ = pa.autotune(p)
+    run = pa.Run(p)
+    run.step(warmup)
+    run.sync()
+    run.set_timing(True)
+    run.step(steps)
+    run.sync()
+    _, sw_ms, n = run.kernel_times()
+    rows = run.results()
+    run.close()
+    att = eqsw * L * L * sum(r.R for r in rows[warmup + 1: warmup + 1 + n])
+    return att / (sw_ms * 1e6), {"slice_bits": p.slice_bits, "threads": p.threads,
+                                 "balance": p.balance}
+
+
+def bench_lattice(args):
+    """Per-lattice-size sweep rates (--mode lattice): one JSON line per (L, coding, R), the
+    sweep kernel only, EqSw and steps as given (the per-L tables and the profiled programs)."""
+    out = []
+    for L in [int(x) for x in args.L.split(",")]:
+        for c in args.coding.split(","):
+            for R in [int(x) for x in args.R.split(",")]:
+                rate, var = lattice_rate(L, pa.SSC if c == "ssc" else pa.MSC, R, args.eqsw,
+                                         args.steps, args.warmup, args.tune)
+                line = {"metric": "sweep-kernel spin-flip attempts/ns", "value": round(rate, 1),
+                        "unit": "spin-flip attempts/ns", "L": L, "coding": c, "R": R,
+                        "eqsweeps": args.eqsw, "steps": args.steps, "warmup": args.warmup,
+                        "variant": var, "tuned": bool(args.tune)}
+                print(json.dumps(line), flush=True)
+                out.append(line)
+    return out
 
 
 def bench_adaptive(args):
@@ -392,7 +434,7 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="plain",
-                    choices=["plain", "batch", "serve", "copy", "config3", "adaptive"])
+                    choices=["plain", "batch", "serve", "copy", "config3", "adaptive", "lattice"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=24, help="beta steps in the CPU baseline sample")
     ap.add_argument("--no-tune", action="store_true",
@@ -409,6 +451,11 @@ def main():
     ap.add_argument("--policy", default="shortest", choices=["shortest", "share"],
                     help="server tick policy (serve mode)")
     ap.add_argument("--check", type=int, default=4, help="served runs re-run on the plain path")
+    ap.add_argument("--L", default="16,32,64", help="lattice mode: lattice sizes")
+    ap.add_argument("--coding", default="ssc,msc", help="lattice mode: codings (ssc, msc)")
+    ap.add_argument("--R", default="20000", help="lattice mode: initial populations")
+    ap.add_argument("--eqsw", type=int, default=50, help="lattice mode: sweeps per step")
+    ap.add_argument("--tune", action="store_true", help="lattice mode: autotune each shape first")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs each PA population on a single "
@@ -430,6 +477,9 @@ def main():
         return
     if args.mode == "adaptive":
         print(json.dumps(bench_adaptive(args)), flush=True)
+        return
+    if args.mode == "lattice":
+        bench_lattice(args)
         return
     name, cu, _ = pa.device_info()
     clock = device_clock_hz()
