@@ -36,8 +36,10 @@
 
 namespace {
 
-// BSD-style getopt as used by the reference (PAisingSSC.cu:338-365): prints
-// "illegal option -- c" / "option requires an argument -- c" and returns '?'.
+// The 4.4BSD getopt(3) (University of California, Berkeley; BSD licence), which the
+// reference carries as its own option parser (PAisingSSC.cu:338-365), kept with its
+// behaviour because the option quirks and messages are part of the command-line contract:
+// prints "illegal option -- c" / "option requires an argument -- c" and returns '?'.
 char* opt_arg;
 int opt_ind = 1, opt_opt;
 
@@ -73,6 +75,8 @@ int get_opt(int nargc, char* const nargv[], const char* ostr) {
   return opt_opt;
 }
 
+// The reference's usage text (PAisingSSC.cu:367-378) word for word for its own options (part
+// of the drop-in contract), followed by the options this build adds.
 void usage(const char* prog) {
   std::cout << " Usage: " << prog << " [options]\n"
             << " Note: all of the options are optional. Default parameter values are listed in the head of the source code. \n"
