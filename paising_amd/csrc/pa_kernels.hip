@@ -1591,8 +1591,8 @@ __global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict
 // part and is evaluated once (the last block is kept in registers).  A workgroup holds
 // RPW = NT / (32 P) replicas of one run, each in its own N-byte LDS slot (several small
 // lattices per workgroup, or one large lattice over 32 P lanes).  The four dwords of a block
-// are straight-line code: a word outside [0, n_q) reads dword 0 and stores to a dummy LDS
-// word, so their LDS reads overlap.  Site geometry (horizontal neighbour dword and byte
+// are straight-line code up to the store: a word outside [0, n_q) reads dword 0 and stores
+// nothing.  Site geometry (horizontal neighbour dword and byte
 // selector, rows above and below) comes from a per-workgroup table of (k, q) entries when
 // P = 1 (at most 12 dwords per quad), else from the dword's offset.  Dword update,
 // thresholds and acceptance as in k_sweep_ssc_q.
@@ -1615,7 +1615,7 @@ __device__ __forceinline__ void ssc_p_geo(int o, int h, int L, int Ld2, int Nh, 
 }
 
 template <bool WIDE, bool TAB>
-__device__ __forceinline__ void ssc_p_sweeps(uint8_t* lat8, const uint4* geo, uint32_t* dummy,
+__device__ __forceinline__ void ssc_p_sweeps(uint8_t* lat8, const uint4* geo,
                                              const RunDesc& d, uint32_t nsw, uint64_t seed,
                                              uint64_t sub, int q, int part, int P, int nq,
                                              bool walk, const uint8_t* tb, const uint64_t* tabw) {
@@ -1685,8 +1685,7 @@ __device__ __forceinline__ void ssc_p_sweeps(uint8_t* lat8, const uint4* geo, ui
                      ((uint64_t)v2 < tabw[((t4 >> 16) & 0xFFu) >> 2] ? 4u : 0u) |
                      ((uint64_t)v3 < tabw[(t4 >> 24) >> 2] ? 8u : 0u);
             }
-            uint32_t* dst = ok ? (uint32_t*)(own8 + o) : dummy;
-            *dst = S ^ (__umul24(bits, 0x204081u) & 0x01010101u);
+            if (ok) *(uint32_t*)(own8 + o) = S ^ (__umul24(bits, 0x204081u) & 0x01010101u);
           }
         }
       }
@@ -1753,7 +1752,6 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   __shared__ uint32_t tabn[10];
   __shared__ uint64_t tabw[10];
   __shared__ int accE[32], accM[32];
-  __shared__ uint32_t dummy;
   if (threadIdx.x < 10) {
     const int i = (int)threadIdx.x, up = i >= 5, c = up ? i - 5 : i;
     const int prod = (2 * up - 1) * (2 * c - 4);
@@ -1769,11 +1767,11 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   __syncthreads();
   const uint8_t* tb = (const uint8_t*)tabn;
   if (wide) {
-    if (tab) ssc_p_sweeps<true, true>(lat8, geo, &dummy, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
-    else ssc_p_sweeps<true, false>(lat8, geo, &dummy, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
+    if (tab) ssc_p_sweeps<true, true>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
+    else ssc_p_sweeps<true, false>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
   } else {
-    if (tab) ssc_p_sweeps<false, true>(lat8, geo, &dummy, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
-    else ssc_p_sweeps<false, false>(lat8, geo, &dummy, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
+    if (tab) ssc_p_sweeps<false, true>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
+    else ssc_p_sweeps<false, false>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
   }
   // back to +1 / -1 bytes for the energies (energyKer SSC:195-215) and the store
   if (live)
