@@ -1702,6 +1702,66 @@ __device__ __forceinline__ void ssc_p_sweeps(uint8_t* lat8, const uint4* geo,
   }
 }
 
+// One part and at most one dword per quad and half sweep (N/2 <= 128, L <= 16): half sweep
+// j takes word j of the four streams, so each Philox block serves four half sweeps (two
+// sweeps) and its word index is known at compile time; the dword's geometry stays in
+// registers.  Same update, thresholds and ordering (wavefront-scope fences) as the one-part
+// walk of ssc_p_sweeps.
+template <bool WIDE>
+__device__ __forceinline__ void ssc_p_sweeps_nq1(uint8_t* lat8, const RunDesc& d, uint32_t nsw,
+                                                 uint64_t seed, uint64_t sub, int q, bool walk,
+                                                 const uint8_t* tb, const uint64_t* tabw) {
+  const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh;
+  const int o = 4 * q;
+  int hx[2], up, dn;
+  uint32_t sel[2];
+  ssc_p_geo(o, 0, L, Ld2, Nh, d.ld2_magic, hx[0], up, dn, sel[0]);
+  ssc_p_geo(o, 1, L, Ld2, Nh, d.ld2_magic, hx[1], up, dn, sel[1]);
+  for (uint32_t s = 0; s < nsw; s += 2) {
+    U4 c0{0, 0, 0, 0}, c1{0, 0, 0, 0}, c2{0, 0, 0, 0}, c3{0, 0, 0, 0};
+    if (walk) {
+      c0 = stream_block(seed, sub + 0, s >> 1);
+      c1 = stream_block(seed, sub + 1, s >> 1);
+      c2 = stream_block(seed, sub + 2, s >> 1);
+      c3 = stream_block(seed, sub + 3, s >> 1);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (s + (i >> 1) >= nsw) break;  // an odd sweep count ends mid-block (uniform)
+      const int h = i & 1;
+      uint8_t* own8 = lat8 + h * Nh;
+      const uint8_t* oth8 = lat8 + (h ^ 1) * Nh;
+      if (walk) {
+        const uint32_t S = *(const uint32_t*)(own8 + o);
+        const uint32_t D0 = *(const uint32_t*)(oth8 + o);
+        const uint32_t Dx = *(const uint32_t*)(oth8 + hx[h]);
+        const uint32_t Du = *(const uint32_t*)(oth8 + up);
+        const uint32_t Dd = *(const uint32_t*)(oth8 + dn);
+        const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, sel[h]);
+        const uint32_t t4 = (D0 + Dh + Du + Dd + (S << 2) + S) << 2;
+        const uint32_t v0 = u4_word(c0, i), v1 = u4_word(c1, i), v2 = u4_word(c2, i),
+                       v3 = u4_word(c3, i);
+        uint32_t bits;
+        if constexpr (!WIDE) {
+          bits = ssc_accept4(*(const uint32_t*)(tb + (t4 & 0xFFu)), v0,
+                             *(const uint32_t*)(tb + ((t4 >> 8) & 0xFFu)), v1,
+                             *(const uint32_t*)(tb + ((t4 >> 16) & 0xFFu)), v2,
+                             *(const uint32_t*)(tb + (t4 >> 24)), v3);
+        } else {
+          bits = ((uint64_t)v0 < tabw[(t4 & 0xFFu) >> 2] ? 1u : 0u) |
+                 ((uint64_t)v1 < tabw[((t4 >> 8) & 0xFFu) >> 2] ? 2u : 0u) |
+                 ((uint64_t)v2 < tabw[((t4 >> 16) & 0xFFu) >> 2] ? 4u : 0u) |
+                 ((uint64_t)v3 < tabw[(t4 >> 24) >> 2] ? 8u : 0u);
+        }
+        *(uint32_t*)(own8 + o) = S ^ (__umul24(bits, 0x204081u) & 0x01010101u);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+}
+
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__ offs, int nruns) {
   const int* off = offs + kKSweep * (nruns + 1);
@@ -1766,7 +1826,10 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   const bool walk = live && nq > 0;
   __syncthreads();
   const uint8_t* tb = (const uint8_t*)tabn;
-  if (wide) {
+  if (tab && Nh <= 128) {
+    if (wide) ssc_p_sweeps_nq1<true>(lat8, d, d.eqsweeps, d.seed, sub, q, walk, tb, tabw);
+    else ssc_p_sweeps_nq1<false>(lat8, d, d.eqsweeps, d.seed, sub, q, walk, tb, tabw);
+  } else if (wide) {
     if (tab) ssc_p_sweeps<true, true>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
     else ssc_p_sweeps<true, false>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, part, P, nq, walk, tb, tabw);
   } else {
