@@ -52,12 +52,16 @@ N = L * L
 #     2 v_bitop3} / 4 = 10), the bit-sliced neighbour count and acceptance (15 bitwise ops)
 #     and two bit reversals (27 ops);
 #   SSC, per site, in the packed-byte form of the sweep (four sites of a row per dword): a
-#     quarter Philox block (10); per site a byte extract of the threshold index, a compare
-#     and an add-with-carry into the flip mask (3); per dword of four sites the horizontal
-#     byte permute, the count-and-index adds (3 + a shift), the flip-mask spread (1) and the
-#     masked xor (1) = 7, i.e. 1.75 per site: 14.75 ops.  (The per-site form counted 17:
-#     neighbour sum 3, product, table index, compare, select and negate.)
-MSC_OPS_PER_BIT, MSC_OPS_PER_WORD, SSC_OPS_PER_SITE = 5, 27, 14.75
+#     quarter Philox block, where the four streams of a dword share a block index and
+#     (nearly always) the high word of their subsequence, so the first round's product and
+#     xor of those and the second round's product of the shared third word are evaluated
+#     once per four blocks: (160 - 9) / 16 = 9.4375; per site a byte extract of the threshold
+#     index, a compare and an add-with-carry into the flip mask (3); per dword of four
+#     sites the horizontal byte permute, the count-and-index adds (3 + a shift), the
+#     flip-mask spread (1) and the masked xor (1) = 7, i.e. 1.75 per site: 14.1875 ops.
+#     (The per-site form counted 17: a quarter block 10, neighbour sum 3, product, table
+#     index, compare, select and negate.)
+MSC_OPS_PER_BIT, MSC_OPS_PER_WORD, SSC_OPS_PER_SITE = 5, 27, 14.1875
 SSC_OPS_PER_SITE_SITEWISE = 17  # the other SSC paths (a variant with a forced block shape)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 

@@ -1521,10 +1521,9 @@ __global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict
       for (int i = 0; i < MB; ++i) {
         if ((int)threadIdx.x + i * NT >= T) continue;
         const uint64_t blk = (uint64_t)(2 * s + h) * nb + m[i];
-        const U4 rb0 = stream_block(seed, sub0 + 4 * q[i] + 0, blk);
-        const U4 rb1 = stream_block(seed, sub0 + 4 * q[i] + 1, blk);
-        const U4 rb2 = stream_block(seed, sub0 + 4 * q[i] + 2, blk);
-        const U4 rb3 = stream_block(seed, sub0 + 4 * q[i] + 3, blk);
+        U4 rq[4];
+        stream_block_quad(seed, sub0 + 4 * q[i], blk, rq);
+        const U4 rb0 = rq[0], rb1 = rq[1], rb2 = rq[2], rb3 = rq[3];
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) {
           const int o = o0[i] + 128 * jj;
@@ -1636,10 +1635,12 @@ __device__ __forceinline__ void ssc_p_sweeps(uint8_t* lat8, const uint4* geo,
         if (b >= (uint32_t)P && b - (uint32_t)P >= bA) b -= (uint32_t)P;
         for (; b <= bB; b += (uint32_t)P) {
           if (b != cb) {
-            c0 = stream_block(seed, sub + 0, b);
-            c1 = stream_block(seed, sub + 1, b);
-            c2 = stream_block(seed, sub + 2, b);
-            c3 = stream_block(seed, sub + 3, b);
+            U4 rq[4];
+            stream_block_quad(seed, sub, b, rq);
+            c0 = rq[0];
+            c1 = rq[1];
+            c2 = rq[2];
+            c3 = rq[3];
             cb = b;
           }
           const int k0 = (int)(4 * b - w0);
@@ -1720,10 +1721,12 @@ __device__ __forceinline__ void ssc_p_sweeps_nq1(uint8_t* lat8, const RunDesc& d
   for (uint32_t s = 0; s < nsw; s += 2) {
     U4 c0{0, 0, 0, 0}, c1{0, 0, 0, 0}, c2{0, 0, 0, 0}, c3{0, 0, 0, 0};
     if (walk) {
-      c0 = stream_block(seed, sub + 0, s >> 1);
-      c1 = stream_block(seed, sub + 1, s >> 1);
-      c2 = stream_block(seed, sub + 2, s >> 1);
-      c3 = stream_block(seed, sub + 3, s >> 1);
+      U4 rq[4];
+      stream_block_quad(seed, sub, s >> 1, rq);
+      c0 = rq[0];
+      c1 = rq[1];
+      c2 = rq[2];
+      c3 = rq[3];
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

@@ -76,6 +76,25 @@ PA_HD U4 stream_block(uint64_t seed, uint64_t sub, uint64_t blk) {
   return philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
+// Block `blk` of the four subsequences sub .. sub+3.  When they share their high 32 bits
+// (all but three in 2^32 cases) the four counters differ only in their third word, so the
+// first round's product of the block index and xor of the high word, and the second round's
+// product of the third word, are common to the four and evaluated once; otherwise each
+// block is evaluated on its own.  The results equal four stream_block calls.
+PA_HD void stream_block_quad(uint64_t seed, uint64_t sub, uint64_t blk, U4 (&o)[4]) {
+  const uint32_t lo = (uint32_t)sub, hi = (uint32_t)(sub >> 32);
+  if (lo <= 0xFFFFFFFCu) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int j = 0; j < 4; ++j)
+      o[j] = philox10(U4{(uint32_t)blk, (uint32_t)(blk >> 32), lo + (uint32_t)j, hi},
+                      (uint32_t)seed, (uint32_t)(seed >> 32));
+  } else {
+    for (int j = 0; j < 4; ++j) o[j] = stream_block(seed, sub + (uint64_t)j, blk);
+  }
+}
+
 PA_HD uint32_t u4_word(const U4& v, int i) {
   return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
