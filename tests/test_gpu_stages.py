@@ -72,6 +72,28 @@ def test_ssc_sweep_stage(L, beta, uniform, nt):
     assert (IE == rIE).all() and (M == rM).all()
 
 
+@pytest.mark.parametrize("L,R,sb", [(64, 2100, 0), (48, 37, 0), (16, 37, 2), (80, 37, 0)])
+def test_ssc_sweep_stage_subsequence_carry(L, R, sb):
+    """Packed SSC sweeps evaluate the Philox blocks of four adjacent subsequences together
+    and share work when the four have the same high 32 bits; a counter starting just below
+    2^32 puts some stream quads across that boundary (their blocks are evaluated one by
+    one) and the rest beyond it.  L=64 with 2100 replicas takes the aligned packed kernel,
+    L=48 the one-part walk, L=16 its one-dword walk, L=80 two parts per quad."""
+    rng = np.random.default_rng(L + R)
+    IS, sweeps, beta = (1 << 32) - 130, 2, 0.44
+    p = params(L, pa.SSC, Rinit=R, eqsweeps=sweeps, slice_bits=sb)
+    pop = ssc_pop(L, R, rng)
+    print(f"case SSC sweep across 2^32 L={L} R={R} sb={sb}", flush=True)
+    got, IE, M = pa.stage_sweep(p, pop, R, IS, beta)
+    ref = np.ascontiguousarray(pop.copy())
+    lib.ora_ssc_sweep(L, R, P(ref), sweeps, p.seed, IS, beta, UMAP[0])
+    assert (got == ref).all()
+    rIE = np.zeros(R, np.int32)
+    rM = np.zeros(R, np.int32)
+    lib.ora_ssc_energy(L, R, P(ref), P(rIE), P(rM))
+    assert (IE == rIE).all() and (M == rM).all()
+
+
 @pytest.mark.parametrize("L,beta,uniform,nt", [(16, 0.44, 0, 0), (24, 1.0, 1, 0), (48, 0.5, 0, 0),
                                                (80, 0.44, 1, 0), (144, 0.44, 0, 256),
                                                (112, 0.7, 0, 128), (64, 0.44, 0, 0)])
