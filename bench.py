@@ -57,11 +57,12 @@ N = L * L
 #     xor of those and the second round's product of the shared third word are evaluated
 #     once per four blocks: (160 - 9) / 16 = 9.4375; per site a byte extract of the threshold
 #     index, a compare and an add-with-carry into the flip mask (3); per dword of four
-#     sites the horizontal byte permute, the count-and-index adds (3 + a shift), the
-#     flip-mask spread (1) and the masked xor (1) = 7, i.e. 1.75 per site: 14.1875 ops.
+#     sites the horizontal byte permute, the count-and-index adds (3: with up spins held as
+#     byte 4 the sum is the threshold's byte offset), the flip-mask spread (1) and the
+#     masked xor (1) = 6, i.e. 1.5 per site: 13.9375 ops.
 #     (The per-site form counted 17: a quarter block 10, neighbour sum 3, product, table
 #     index, compare, select and negate.)
-MSC_OPS_PER_BIT, MSC_OPS_PER_WORD, SSC_OPS_PER_SITE = 5, 27, 14.1875
+MSC_OPS_PER_BIT, MSC_OPS_PER_WORD, SSC_OPS_PER_SITE = 5, 27, 13.9375
 SSC_OPS_PER_SITE_SITEWISE = 17  # the other SSC paths (a variant with a forced block shape)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 

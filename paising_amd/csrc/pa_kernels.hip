@@ -1410,16 +1410,10 @@ __global__ __launch_bounds__(NT) void k_sweep_ssc(const RunDesc* __restrict__ ru
 // 4q..4q+3, Philox block m), and for each word jj of the block the four sites
 // 4q + j + 128 (4m + jj), j = 0..3, are four consecutive sites of one row (L/2 and 4q + 128 k
 // are multiples of 4), i.e. one aligned dword of the sublattice in LDS.  Spins are held in
-// LDS as bytes 0 (down) / 1 (up), so the four neighbours of all four sites are four dwords
-// (same index, above, below, and the horizontal neighbours taken from the adjacent group by
-// one byte permute), the up-neighbour counts c are four bytes of one add, and the threshold
-// index c + 5 s (s the own spin: 10 classes, threshold by the reference's product
-// (2s-1)(2c-4)) four bytes of another.  Per site: a byte extract, one LDS threshold read, a
-// compare and an add-with-carry that shifts the decision into a 4-bit mask, which one
-// multiply spreads back over the four bytes to flip them.
+// LDS as bytes 0 / 4, so the four neighbours of all four sites are four dwords (same index,
+// above, below, and the horizontal neighbours taken from the adjacent group by one byte
+// permute) and the four sites' threshold offsets four bytes of one sum (ssc_dword_update).
 //
-// Thresholds are compared as v <= V - 1 (v < V for V >= 1; V = 2^32, every move accepted,
-// included).  When a threshold is 0 (beta above about 2.8) the 64-bit form v < V is used.
 // Each thread takes MB = 2 tasks (tid and tid + NT): two tasks' eight Philox blocks in flight
 // ran 10-30 % faster than one task per thread at equal or higher lane occupancy (L = 96..256),
 // four tasks per thread spilled and ran at half speed.
@@ -1450,6 +1444,53 @@ __device__ __forceinline__ uint32_t ssc_accept4(uint32_t T0, uint32_t v0, uint32
 // [x, x+3]: the group after (bytes D0[1..3], Dx[0]) or the group before (Dx[3], D0[0..2]).
 constexpr uint32_t kSelNext = 0x04030201u, kSelPrev = 0x02010007u;
 
+// LDS byte of a site in the packed-byte SSC kernels: 0 (down) or 4 (up).  Conversion
+// from and to the population's int8 +1 / -1 (bit 1 of ~x is the sign of x):
+__device__ __forceinline__ uint32_t ssc_to_bytes4(uint32_t g) { return (~g << 1) & 0x04040404u; }
+__device__ __forceinline__ uint32_t ssc_from_bytes4(uint32_t x) { return ~((x >> 2) * 0xFEu); }
+
+// Update of one dword = four sites of a row, own bytes S, neighbour bytes D0 (same index),
+// Dh (horizontal, after the permute), Du, Dd (rows above and below), Philox words v0..v3.
+// With up = 4 the byte sum D0 + Dh + Du + Dd + 5 S = 4 (c + 5 s), c the up neighbours and s
+// the own spin, is already the byte offset of the site's threshold class in the table tb (10
+// classes, thresholds by the reference's product (2s-1)(2c-4)); no byte exceeds 36.  Per site a
+// byte extract, one LDS threshold read, a compare and an add-with-carry that shifts the
+// decision into a 4-bit mask; one multiply spreads the mask to bit 2 of the four bytes
+// (b x (2^2 + 2^9 + 2^16 + 2^23): bit j lands at 8 j + 2, no carries) to flip them.
+// Thresholds are compared as v <= V - 1 (v < V for V >= 1; V = 2^32, every move accepted,
+// included); WIDE (a threshold 0, beta above about 2.8) uses the 64-bit form v < V.
+template <bool WIDE>
+__device__ __forceinline__ uint32_t ssc_dword_update(uint32_t S, uint32_t D0, uint32_t Dh,
+                                                     uint32_t Du, uint32_t Dd, uint32_t v0,
+                                                     uint32_t v1, uint32_t v2, uint32_t v3,
+                                                     const uint8_t* tb, const uint64_t* tabw) {
+  // three instructions (the compiler splits 5 S into a shift and an add of its own)
+  uint32_t t4;
+  asm("v_lshl_add_u32 %[t], %[s], 2, %[d0]\n\t"
+      "v_add3_u32 %[t], %[t], %[s], %[du]\n\t"
+      "v_add3_u32 %[t], %[t], %[dd], %[dh]"
+      : [t] "=&v"(t4)
+      : [s] "v"(S), [d0] "v"(D0), [du] "v"(Du), [dd] "v"(Dd), [dh] "v"(Dh));
+  uint32_t bits;
+  if constexpr (!WIDE) {
+    bits = ssc_accept4(*(const uint32_t*)(tb + (t4 & 0xFFu)), v0,
+                       *(const uint32_t*)(tb + ((t4 >> 8) & 0xFFu)), v1,
+                       *(const uint32_t*)(tb + ((t4 >> 16) & 0xFFu)), v2,
+                       *(const uint32_t*)(tb + (t4 >> 24)), v3);
+  } else {
+    bits = ((uint64_t)v0 < tabw[(t4 & 0xFFu) >> 2] ? 1u : 0u) |
+           ((uint64_t)v1 < tabw[((t4 >> 8) & 0xFFu) >> 2] ? 2u : 0u) |
+           ((uint64_t)v2 < tabw[((t4 >> 16) & 0xFFu) >> 2] ? 4u : 0u) |
+           ((uint64_t)v3 < tabw[(t4 >> 24) >> 2] ? 8u : 0u);
+  }
+  return S ^ (__umul24(bits, 0x810204u) & 0x04040404u);
+}
+
+template <bool B>
+struct BoolC {
+  static constexpr bool value = B;
+};
+
 template <int NTMAX, int MB>
 __global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict__ runs,
                                                       const int32_t* __restrict__ offs, int nruns) {
@@ -1465,8 +1506,7 @@ __global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict
   const int L = d.L, Ld2 = d.Ld2, Nh = d.Nh, N = d.N;
   const int step = st->step;
   uint32_t* g = (uint32_t*)((int8_t*)d.pop[st->buf ^ 1] + (size_t)B * N);
-  // +1 -> 1, -1 -> 0: bit 1 of a byte is the sign
-  for (int i = threadIdx.x; i < N / 4; i += NT) latq[i] = (~g[i] >> 1) & 0x01010101u;
+  for (int i = threadIdx.x; i < N / 4; i += NT) latq[i] = ssc_to_bytes4(g[i]);
   const uint64_t V1 = d.thr[3 * step + 0], V4 = d.thr[3 * step + 1], V8 = d.thr[3 * step + 2];
   __shared__ uint32_t tabn[10];
   __shared__ uint64_t tabw[10];
@@ -1477,7 +1517,7 @@ __global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict
     tabw[i] = V;
     tabn[i] = (uint32_t)(V - 1);
   }
-  const bool wide = V4 == 0 || V8 == 0;
+  const bool wide = __builtin_amdgcn_readfirstlane((int)(V4 == 0 || V8 == 0)) != 0;
   const uint64_t seed = d.seed;
   const uint64_t sub0 = st->IS + (uint64_t)st->R + (uint64_t)B * kStreamsPerUnit;
   // the thread's tasks and their fixed geometry (byte offsets within a sublattice)
@@ -1512,52 +1552,42 @@ __global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict
   }
   __syncthreads();
   const uint8_t* tb = (const uint8_t*)tabn;
-  for (uint32_t s = 0; s < d.eqsweeps; ++s) {
+  // the threshold form is uniform: one loop per form, no branch per dword
+  auto sweeps = [&](auto wide_c) __attribute__((always_inline)) {
+    constexpr bool WIDE = decltype(wide_c)::value;
+    for (uint32_t s = 0; s < d.eqsweeps; ++s) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      uint8_t* own8 = (uint8_t*)latq + h * Nh;
-      const uint8_t* oth8 = (const uint8_t*)latq + (h ^ 1) * Nh;
+      for (int h = 0; h < 2; ++h) {
+        uint8_t* own8 = (uint8_t*)latq + h * Nh;
+        const uint8_t* oth8 = (const uint8_t*)latq + (h ^ 1) * Nh;
 #pragma unroll
-      for (int i = 0; i < MB; ++i) {
-        if ((int)threadIdx.x + i * NT >= T) continue;
-        const uint64_t blk = (uint64_t)(2 * s + h) * nb + m[i];
-        U4 rq[4];
-        stream_block_quad(seed, sub0 + 4 * q[i], blk, rq);
-        const U4 rb0 = rq[0], rb1 = rq[1], rb2 = rq[2], rb3 = rq[3];
+        for (int i = 0; i < MB; ++i) {
+          if ((int)threadIdx.x + i * NT >= T) continue;
+          const uint64_t blk = (uint64_t)(2 * s + h) * nb + m[i];
+          U4 rq[4];
+          stream_block_quad(seed, sub0 + 4 * q[i], blk, rq);
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          const int o = o0[i] + 128 * jj;
-          const uint32_t S = *(const uint32_t*)(own8 + o);
-          const uint32_t D0 = *(const uint32_t*)(oth8 + o);
-          const uint32_t Dx = *(const uint32_t*)(oth8 + hx[i][h][jj]);
-          const uint32_t Du = *(const uint32_t*)(oth8 + (jj == 0 ? up0[i] : o - Ld2));
-          const uint32_t Dd = *(const uint32_t*)(oth8 + (jj == 3 ? dn3[i] : o + Ld2));
-          const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, sel[i][h][jj]);
-          // threshold index x 4 per byte: (c + 5 s) << 2 <= 36
-          const uint32_t t4 = (D0 + Dh + Du + Dd + (S << 2) + S) << 2;
-          const uint32_t v0 = u4_word(rb0, jj), v1 = u4_word(rb1, jj), v2 = u4_word(rb2, jj),
-                         v3 = u4_word(rb3, jj);
-          uint32_t bits;
-          if (!wide) {
-            bits = ssc_accept4(*(const uint32_t*)(tb + (t4 & 0xFFu)), v0,
-                               *(const uint32_t*)(tb + ((t4 >> 8) & 0xFFu)), v1,
-                               *(const uint32_t*)(tb + ((t4 >> 16) & 0xFFu)), v2,
-                               *(const uint32_t*)(tb + (t4 >> 24)), v3);
-          } else {
-            bits = ((uint64_t)v0 < tabw[(t4 & 0xFFu) >> 2] ? 1u : 0u) |
-                   ((uint64_t)v1 < tabw[((t4 >> 8) & 0xFFu) >> 2] ? 2u : 0u) |
-                   ((uint64_t)v2 < tabw[((t4 >> 16) & 0xFFu) >> 2] ? 4u : 0u) |
-                   ((uint64_t)v3 < tabw[(t4 >> 24) >> 2] ? 8u : 0u);
+          for (int jj = 0; jj < 4; ++jj) {
+            const int o = o0[i] + 128 * jj;
+            const uint32_t S = *(const uint32_t*)(own8 + o);
+            const uint32_t D0 = *(const uint32_t*)(oth8 + o);
+            const uint32_t Dx = *(const uint32_t*)(oth8 + hx[i][h][jj]);
+            const uint32_t Du = *(const uint32_t*)(oth8 + (jj == 0 ? up0[i] : o - Ld2));
+            const uint32_t Dd = *(const uint32_t*)(oth8 + (jj == 3 ? dn3[i] : o + Ld2));
+            const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, sel[i][h][jj]);
+            *(uint32_t*)(own8 + o) = ssc_dword_update<WIDE>(
+                S, D0, Dh, Du, Dd, u4_word(rq[0], jj), u4_word(rq[1], jj), u4_word(rq[2], jj),
+                u4_word(rq[3], jj), tb, tabw);
           }
-          // bits b3..b0 -> bytes: b x (1 + 2^7 + 2^14 + 2^21) puts bit j at 8 j, no carries
-          *(uint32_t*)(own8 + o) = S ^ (__umul24(bits, 0x204081u) & 0x01010101u);
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
+  };
+  if (wide) sweeps(BoolC<true>{});
+  else sweeps(BoolC<false>{});
   // back to +1 / -1 bytes for the energies (energyKer SSC:195-215) and the store
-  for (int i = threadIdx.x; i < N / 4; i += NT) latq[i] = ~(latq[i] * 0xFEu);
+  for (int i = threadIdx.x; i < N / 4; i += NT) latq[i] = ssc_from_bytes4(latq[i]);
   __syncthreads();
   const int8_t* lat8 = (const int8_t*)latq;
   int e = 0, mm = 0;
@@ -1670,23 +1700,10 @@ __device__ __forceinline__ void ssc_p_sweeps(uint8_t* lat8, const uint4* geo,
             const uint32_t Du = *(const uint32_t*)(oth8 + up);
             const uint32_t Dd = *(const uint32_t*)(oth8 + dn);
             const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, sel);
-            // threshold index x 4 per byte: (c + 5 s) << 2 <= 36
-            const uint32_t t4 = (D0 + Dh + Du + Dd + (S << 2) + S) << 2;
-            const uint32_t v0 = u4_word(c0, i), v1 = u4_word(c1, i), v2 = u4_word(c2, i),
-                           v3 = u4_word(c3, i);
-            uint32_t bits;
-            if constexpr (!WIDE) {
-              bits = ssc_accept4(*(const uint32_t*)(tb + (t4 & 0xFFu)), v0,
-                                 *(const uint32_t*)(tb + ((t4 >> 8) & 0xFFu)), v1,
-                                 *(const uint32_t*)(tb + ((t4 >> 16) & 0xFFu)), v2,
-                                 *(const uint32_t*)(tb + (t4 >> 24)), v3);
-            } else {
-              bits = ((uint64_t)v0 < tabw[(t4 & 0xFFu) >> 2] ? 1u : 0u) |
-                     ((uint64_t)v1 < tabw[((t4 >> 8) & 0xFFu) >> 2] ? 2u : 0u) |
-                     ((uint64_t)v2 < tabw[((t4 >> 16) & 0xFFu) >> 2] ? 4u : 0u) |
-                     ((uint64_t)v3 < tabw[(t4 >> 24) >> 2] ? 8u : 0u);
-            }
-            if (ok) *(uint32_t*)(own8 + o) = S ^ (__umul24(bits, 0x204081u) & 0x01010101u);
+            const uint32_t S2 =
+                ssc_dword_update<WIDE>(S, D0, Dh, Du, Dd, u4_word(c0, i), u4_word(c1, i),
+                                       u4_word(c2, i), u4_word(c3, i), tb, tabw);
+            if (ok) *(uint32_t*)(own8 + o) = S2;
           }
         }
       }
@@ -1741,22 +1758,9 @@ __device__ __forceinline__ void ssc_p_sweeps_nq1(uint8_t* lat8, const RunDesc& d
         const uint32_t Du = *(const uint32_t*)(oth8 + up);
         const uint32_t Dd = *(const uint32_t*)(oth8 + dn);
         const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, sel[h]);
-        const uint32_t t4 = (D0 + Dh + Du + Dd + (S << 2) + S) << 2;
-        const uint32_t v0 = u4_word(c0, i), v1 = u4_word(c1, i), v2 = u4_word(c2, i),
-                       v3 = u4_word(c3, i);
-        uint32_t bits;
-        if constexpr (!WIDE) {
-          bits = ssc_accept4(*(const uint32_t*)(tb + (t4 & 0xFFu)), v0,
-                             *(const uint32_t*)(tb + ((t4 >> 8) & 0xFFu)), v1,
-                             *(const uint32_t*)(tb + ((t4 >> 16) & 0xFFu)), v2,
-                             *(const uint32_t*)(tb + (t4 >> 24)), v3);
-        } else {
-          bits = ((uint64_t)v0 < tabw[(t4 & 0xFFu) >> 2] ? 1u : 0u) |
-                 ((uint64_t)v1 < tabw[((t4 >> 8) & 0xFFu) >> 2] ? 2u : 0u) |
-                 ((uint64_t)v2 < tabw[((t4 >> 16) & 0xFFu) >> 2] ? 4u : 0u) |
-                 ((uint64_t)v3 < tabw[(t4 >> 24) >> 2] ? 8u : 0u);
-        }
-        *(uint32_t*)(own8 + o) = S ^ (__umul24(bits, 0x204081u) & 0x01010101u);
+        *(uint32_t*)(own8 + o) =
+            ssc_dword_update<WIDE>(S, D0, Dh, Du, Dd, u4_word(c0, i), u4_word(c1, i),
+                                   u4_word(c2, i), u4_word(c3, i), tb, tabw);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -1791,9 +1795,8 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   const int q = lr & 31, part = lr >> 5;
   uint8_t* lat8 = (uint8_t*)latq + (size_t)(live ? slot : 0) * N;
   uint32_t* g = (uint32_t*)((int8_t*)d.pop[st->buf ^ 1] + (size_t)(live ? B : 0) * N);
-  // +1 -> 1, -1 -> 0: bit 1 of a byte is the sign
   if (live)
-    for (int i = lr; i < N / 4; i += lanes) ((uint32_t*)lat8)[i] = (~g[i] >> 1) & 0x01010101u;
+    for (int i = lr; i < N / 4; i += lanes) ((uint32_t*)lat8)[i] = ssc_to_bytes4(g[i]);
   const bool tab = P == 1;
   uint4* geo = (uint4*)((uint8_t*)latq + (size_t)RPW * N);
   if (tab) {
@@ -1841,7 +1844,7 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   }
   // back to +1 / -1 bytes for the energies (energyKer SSC:195-215) and the store
   if (live)
-    for (int i = lr; i < N / 4; i += lanes) ((uint32_t*)lat8)[i] = ~(((uint32_t*)lat8)[i] * 0xFEu);
+    for (int i = lr; i < N / 4; i += lanes) ((uint32_t*)lat8)[i] = ssc_from_bytes4(((uint32_t*)lat8)[i]);
   __syncthreads();
   int e = 0, mm = 0;
   if (live) {
