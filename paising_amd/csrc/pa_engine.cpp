@@ -643,6 +643,15 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     }
     h.sw = sw;
     h.half = half;
+    // 5 to 13 sites per stream and half sweep (L = 40, 48, 56; words of 32 bits or more):
+    // 512-thread workgroups where the population fills the balanced sweep, 1024 (eight
+    // parts per stream, more waves per unit) where it has fewer units than CUs (R = 20000 /
+    // 5000, attempts/ns, against the 256-thread default: L = 40 3034 vs 2774, 1704 vs 1221;
+    // 48 3390 vs 3027, 2113 vs 1471; 56 3358 vs 3065)
+    if (!p.threads && h.w >= 32 && Nh > 4 * 128 && Nh <= 13 * 128) {
+      const int64_t units = ((int64_t)ceil_div(p.Rinit, h.w) + co_work / h.N) * (h.w / sw);
+      h.nt = units * 9 / 10 >= device_cus() ? 512 : 1024;
+    }
     if (half) {
       h.nt = 1024;
       h.path = 1;
