@@ -577,7 +577,9 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     if (h.sweep_lds() + 1024 > kLdsLimit)
       return fail(PA_ERR_INVALID, "packed-byte SSC workgroup exceeds LDS");
     h.nt = nt8;
-    h.path = 8;
+    // one part and 257..384 sites per sublattice (L = 24): the two-sweep step
+    // (k_sweep_ssc_p2, at most 512 threads)
+    h.path = h.ssc_p == 1 && nt8 <= 512 && Nh > 256 && Nh <= 384 ? 9 : 8;
   } else if (h.nt != 128 && h.nt != 256 && h.nt != 512 && !(h.nt == 1024 && p.coding == PA_MSC)) {
     // 1024-thread workgroups (MSC only): twice the waves per CU for lattices whose slice
     // leaves room for one workgroup per CU
@@ -1062,7 +1064,7 @@ struct Batch {
           const size_t l = h.p.coding == PA_SSC ? (size_t)h.N
                            : (size_t)(h.half ? h.N / 2 : h.N) * (h.sw / 8);
           lds = std::max(lds, l);
-          lds8 = std::max(lds8, h.path == 8 ? h.sweep_lds() : 0);
+          lds8 = std::max(lds8, h.path == 8 || h.path == 9 ? h.sweep_lds() : 0);
         }
         go[(size_t)kKUnits * (m + 1) + m] = acc;
         go[(size_t)kKSweep * (m + 1) + m] = acc8;
@@ -1206,8 +1208,8 @@ struct Batch {
     else if (g.bal_workers > 0)
       HIPCHECK(pa_launch_step(g.bal_kind, g.v.coding, g.v.w, g.v.sw, g.v.nt, g.bal_workers, g.bal_lds, s,
                               g.desc_d, g.offs_d, (int)g.members.size()));
-    else if (g.v.coding == PA_SSC && g.v.path == 8)
-      HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, 8, g.v.nt, g.sweep_blocks, g.sweep_lds, s,
+    else if (g.v.coding == PA_SSC && (g.v.path == 8 || g.v.path == 9))
+      HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.path, g.v.nt, g.sweep_blocks, g.sweep_lds, s,
                               g.desc_d, g.offs_d, (int)g.members.size()));
     else
       HIPCHECK(pa_launch_step(4, g.v.coding, g.v.w, g.v.coding == PA_SSC ? g.v.path : g.v.sw, g.v.nt,
@@ -2217,8 +2219,8 @@ extern "C" int pa_stage_sweep_noalloc(const pa_params* p, int R, uint64_t is, do
   if (h.p.coding == PA_MSC && h.half)
     HIPCHECK(pa_launch_step(16, PA_MSC, h.w, 32, 1024, units * (h.w / 32), lds, s, &dev->desc,
                             dev->offs, 1));
-  else if (h.p.coding == PA_SSC && h.path == 8)
-    HIPCHECK(pa_launch_step(4, PA_SSC, 1, 8, h.nt, h.blocks(kKSweep), h.sweep_lds(), s,
+  else if (h.p.coding == PA_SSC && (h.path == 8 || h.path == 9))
+    HIPCHECK(pa_launch_step(4, PA_SSC, 1, h.path, h.nt, h.blocks(kKSweep), h.sweep_lds(), s,
                             &dev->desc, dev->offs, 1));
   else
     HIPCHECK(pa_launch_step(4, h.p.coding, h.w, h.p.coding == PA_SSC ? h.path : h.sw, h.nt,

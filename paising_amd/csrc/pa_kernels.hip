@@ -1769,8 +1769,84 @@ __device__ __forceinline__ void ssc_p_sweeps_nq1(uint8_t* lat8, const RunDesc& d
   }
 }
 
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
-void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__ offs, int nruns) {
+// One part, 257 to 512 sites per sublattice (L = 24, 32): quad q has n_q = C or C - 1
+// dwords per half sweep (C = ceil(N/2 / 128) = 3 or 4), so two sweeps (half sweeps
+// j = 4J .. 4J + 3) take words 4 J n_q .. 4 J n_q + 4 n_q - 1 of its streams, exactly the
+// Philox blocks J n_q + p, p < n_q; which word of which block feeds dword k of half sweep hs is
+// known at compile time for both lane classes (word hs n_q + k of the two sweeps).  Blocks
+// are evaluated in order, each before the first half sweep whose C-class dwords need it; a
+// (C-1)-class lane lags at most one block behind (its block C - 1 is evaluated and unused).
+// Where the classes read different words (MIXED: C = 3 at L = 24) a dword's four words are
+// selected per lane.  Geometry from the per-workgroup table; a dword a lane does not own
+// reads dword 0 and stores nothing.  Same update and wavefront-scope fences as the one-part
+// walk of ssc_p_sweeps, none of its loop control.
+template <bool WIDE, int C, bool MIXED>
+__device__ __forceinline__ void ssc_p_sweeps_c(uint8_t* lat8, const uint4* geo, const RunDesc& d,
+                                               uint32_t nsw, uint64_t seed, uint64_t sub, int q,
+                                               int nq, bool walk, const uint8_t* tb,
+                                               const uint64_t* tabw) {
+  const int Nh = d.Nh;
+  const bool big = nq == C;
+  const uint64_t n64 = (uint64_t)(uint32_t)nq;
+  for (uint32_t s = 0; s < nsw; s += 2) {
+    U4 blk[C][4];
+#pragma unroll
+    for (int hs = 0; hs < 4; ++hs) {
+      if (s + (hs >> 1) >= nsw) continue;  // an odd sweep count ends mid-step (uniform)
+      const int h = hs & 1;
+      uint8_t* own8 = lat8 + h * Nh;
+      const uint8_t* oth8 = lat8 + (h ^ 1) * Nh;
+      if (walk) {
+#pragma unroll
+        for (int p = 0; p < C; ++p)
+          if ((4 * p) / C == hs) {
+            U4 rq[4];
+            stream_block_quad(seed, sub, (uint64_t)(s >> 1) * n64 + p, rq);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) blk[p][j] = rq[j];
+          }
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+          const int wb = hs * C + k, ws = hs * (C - 1) + k;
+          uint32_t v[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t vb = u4_word(blk[wb >> 2][j], wb & 3);
+            if constexpr (MIXED) {
+              if (k < C - 1 && (ws >> 2 != wb >> 2 || (ws & 3) != (wb & 3)))
+                v[j] = big ? vb : u4_word(blk[ws >> 2][j], ws & 3);
+              else
+                v[j] = vb;
+            } else {
+              v[j] = vb;
+            }
+          }
+          // geometry from the table (a dword the lane does not own has dword 0's entry)
+          const uint4 e = geo[k * 32 + q];
+          const int o = (int)e.w;
+          const uint32_t S = *(const uint32_t*)(own8 + o);
+          const uint32_t D0 = *(const uint32_t*)(oth8 + o);
+          const uint32_t Dx = *(const uint32_t*)(oth8 + (h ? (e.x >> 16) : (e.x & 0xFFFFu)));
+          const uint32_t Du = *(const uint32_t*)(oth8 + (e.y & 0xFFFFu));
+          const uint32_t Dd = *(const uint32_t*)(oth8 + (e.y >> 16));
+          const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, h ? kSelSum - e.z : e.z);
+          const uint32_t S2 =
+              ssc_dword_update<WIDE>(S, D0, Dh, Du, Dd, v[0], v[1], v[2], v[3], tb, tabw);
+          if (k < C - 1 || !MIXED || big) *(uint32_t*)(own8 + o) = S2;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+}
+
+// MODE 0: the stream walk of any L % 8 == 0 (k_sweep_ssc_p); MODE 1: two sweeps per step for
+// one part and 257..384 sites per sublattice (k_sweep_ssc_p2, its own register budget).
+template <int MODE>
+__device__ __forceinline__ void ssc_p_body(const RunDesc* __restrict__ runs,
+                                           const int32_t* __restrict__ offs, int nruns) {
   const int* off = offs + kKSweep * (nruns + 1);
   const int r = find_run(off, nruns, blockIdx.x);
   const RunDesc& d = runs[r];
@@ -1780,7 +1856,9 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   const int P = d.ssc_p, lanes = 32 * P, RPW = NT / lanes;
   const int Rn = st->Rnew;
   const int B0 = (blockIdx.x - off[r]) * RPW;
-  if (RPW < 1 || RPW > 32) {  // a launch shape the plan never makes: refuse (host error bit 2)
+  const int Nh0 = d.Nh;
+  if (RPW < 1 || RPW > 32 ||  // a launch shape the plan never makes: refuse (host error bit 2)
+      (MODE == 1 && !(P == 1 && Nh0 > 256 && Nh0 <= 384))) {
     if (threadIdx.x == 0) atomicOr(&st->err, 2);
     return;
   }
@@ -1802,8 +1880,9 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   if (tab) {
     const int nmax = (Nh + 127) >> 7;
     for (int i = threadIdx.x; i < 32 * nmax; i += NT) {
-      const int qq = i & 31, k = i >> 5, o = 4 * qq + 128 * k;
-      if (o < Nh) {
+      const int qq = i & 31, k = i >> 5, o0 = 4 * qq + 128 * k;
+      const int o = o0 < Nh ? o0 : 4 * qq;  // past the quad's last dword: its dword 0
+      {
         int hx0, hx1, up, dn;
         uint32_t sel0, sel1;
         ssc_p_geo(o, 0, L, Ld2, Nh, d.ld2_magic, hx0, up, dn, sel0);
@@ -1832,7 +1911,10 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   const bool walk = live && nq > 0;
   __syncthreads();
   const uint8_t* tb = (const uint8_t*)tabn;
-  if (tab && Nh <= 128) {
+  if constexpr (MODE == 1) {
+    if (wide) ssc_p_sweeps_c<true, 3, true>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, nq, walk, tb, tabw);
+    else ssc_p_sweeps_c<false, 3, true>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, nq, walk, tb, tabw);
+  } else if (tab && Nh <= 128) {
     if (wide) ssc_p_sweeps_nq1<true>(lat8, d, d.eqsweeps, d.seed, sub, q, walk, tb, tabw);
     else ssc_p_sweeps_nq1<false>(lat8, d, d.eqsweeps, d.seed, sub, q, walk, tb, tabw);
   } else if (wide) {
@@ -1876,6 +1958,19 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
     if (lr == 0) { d.IE[B] = accE[slot]; d.M[B] = accM[slot]; }
     for (int i = lr; i < N / 4; i += lanes) g[i] = ((const uint32_t*)lat8)[i];
   }
+}
+
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__ offs, int nruns) {
+  ssc_p_body<0>(runs, offs, nruns);
+}
+
+// Up to 512 threads and 168 VGPRs (three waves per SIMD): the two-sweep step holds two
+// Philox quads and the geometry of up to four dwords, which spill at 80 (six waves, the
+// budget of k_sweep_ssc_p) and at 128.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+void k_sweep_ssc_p2(const RunDesc* __restrict__ runs, const int32_t* __restrict__ offs, int nruns) {
+  ssc_p_body<1>(runs, offs, nruns);
 }
 
 // MSC sweeps of one unit = (group g, slice qs): sweeps [s_begin, s_end) of the step, with the
@@ -2711,6 +2806,7 @@ hipError_t pa_prepare_kernels(void) {
   PA_ALLOW((k_sweep_ssc<512, 5>));
   PA_ALLOW((k_sweep_ssc_q<1024, 2>));
   PA_ALLOW(k_sweep_ssc_p);
+  PA_ALLOW(k_sweep_ssc_p2);
 #define PA_ALLOW_MSC(WT, ST)                          \
   PA_ALLOW((k_sweep_msc_bal<WT, ST, 128, false>));    \
   PA_ALLOW((k_sweep_msc_bal<WT, ST, 256, false>));    \
@@ -2837,10 +2933,12 @@ hipError_t pa_launch_step(int kind, int coding, int w, int sw, int nt, int nbloc
             else if (nt == 128) hipLaunchKernelGGL((k_sweep_ssc<128, 5>), dim3(nblocks), dim3(128), lds, s, runs, offs, nruns);
             else return hipErrorInvalidValue;
           }
-        } else if (sw == 8) {
+        } else if (sw == 8 || sw == 9) {
           // packed-byte sweeps of any L % 8 == 0: RPW = nt / (32 P) replicas per workgroup
-          if (nt < 64 || nt > 1024 || nt % 64) return hipErrorInvalidValue;
-          hipLaunchKernelGGL(k_sweep_ssc_p, dim3(nblocks), dim3(nt), lds, s, runs, offs, nruns);
+          // (9: the two-sweep step of L = 24)
+          if (nt < 64 || nt > (sw == 8 ? 1024 : 512) || nt % 64) return hipErrorInvalidValue;
+          if (sw == 8) hipLaunchKernelGGL(k_sweep_ssc_p, dim3(nblocks), dim3(nt), lds, s, runs, offs, nruns);
+          else hipLaunchKernelGGL(k_sweep_ssc_p2, dim3(nblocks), dim3(nt), lds, s, runs, offs, nruns);
         } else if (sw == 7) {
           // packed-byte sweeps, two tasks per thread; nt is a whole number of waves
           if (nt < 64 || nt > 1024 || nt % 64) return hipErrorInvalidValue;

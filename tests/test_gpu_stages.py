@@ -54,7 +54,11 @@ def msc_pop(L, w, R, rng):
                                                (56, 0.6, 0, 0), (80, 0.44, 1, 0), (112, 1.0, 0, 256),
                                                (144, 0.44, 0, 1024), (208, 0.5, 1, 0), (240, 0.44, 0, 0),
                                                (48, 6.0, 0, 0), (144, 6.0, 1, 0), (320, 0.44, 0, 0),
-                                               (16, 6.0, 0, -256)])
+                                               (16, 6.0, 0, -256),
+                                               # the two-sweep step (L = 24: three or two dwords
+                                               # per quad; L = 32: four), both threshold forms
+                                               (24, 0.44, 0, -256), (24, 6.0, 1, -64),
+                                               (24, 0.7, 1, -512), (32, 6.0, 0, -512)])
 def test_ssc_sweep_stage(L, beta, uniform, nt):
     rng = np.random.default_rng(L + abs(nt))
     R, IS, sweeps = 37, 9876543210, 3
@@ -72,13 +76,15 @@ def test_ssc_sweep_stage(L, beta, uniform, nt):
     assert (IE == rIE).all() and (M == rM).all()
 
 
-@pytest.mark.parametrize("L,R,sb", [(64, 2100, 0), (48, 37, 0), (16, 37, 2), (80, 37, 0)])
+@pytest.mark.parametrize("L,R,sb", [(64, 2100, 0), (48, 37, 0), (16, 37, 2), (80, 37, 0),
+                                    (24, 37, 2), (32, 37, 2)])
 def test_ssc_sweep_stage_subsequence_carry(L, R, sb):
     """Packed SSC sweeps evaluate the Philox blocks of four adjacent subsequences together
     and share work when the four have the same high 32 bits; a counter starting just below
     2^32 puts some stream quads across that boundary (their blocks are evaluated one by
     one) and the rest beyond it.  L=64 with 2100 replicas takes the aligned packed kernel,
-    L=48 the one-part walk, L=16 its one-dword walk, L=80 two parts per quad."""
+    L=48 the one-part walk, L=16 its one-dword walk, L=80 two parts per quad, L=24 and 32
+    the two-sweep step."""
     rng = np.random.default_rng(L + R)
     IS, sweeps, beta = (1 << 32) - 130, 2, 0.44
     p = params(L, pa.SSC, Rinit=R, eqsweeps=sweeps, slice_bits=sb)
