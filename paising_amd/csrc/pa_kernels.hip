@@ -2173,6 +2173,36 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
         }
       }
     }
+  } else if (Nh <= kStreamsPerUnit) {
+    // At most one site per stream and half sweep (L <= 16): half sweep j = 2s + h takes word
+    // j of stream t, so one Philox block serves two sweeps with compile-time word indices, and
+    // the two sites' neighbour indices stay in registers.  Threads t >= N/2 (and those of
+    // other parts) only join the barriers.
+    const int t = (int)threadIdx.x;
+    const bool act = t < Nh;
+    const Site qh[2] = {site_at(act ? t : 0, 0, d), site_at(act ? t : 0, 1, d)};
+    for (uint32_t s0 = s_begin & ~1u; s0 < s_end; s0 += 2) {
+      U4 rb{0, 0, 0, 0};
+      if (act) rb = stream_block(seed, sub0 + t, s0 >> 1);
+#pragma unroll
+      for (int hs = 0; hs < 4; ++hs) {
+        const uint32_t s = s0 + (hs >> 1);
+        if (s < s_begin || s >= s_end) continue;  // uniform: a range starting or ending mid-block
+        const int h = hs & 1;
+        if (act) {
+          S* own = lat + h * Nh;
+          const S* oth = lat + (h ^ 1) * Nh;
+          uint32_t ran[1] = {JA * u4_word(rb, hs) + JC};
+          S c4[1], c8[1];
+          if constexpr (SW >= 16) lcg_masks_split<S>(ran[0], T4, T8, c4[0], c8[0]);
+          else lcg_masks<S, 1>(ran, T4, T8, c4, c8);
+          const Site& q = qh[h];
+          const S sp = own[q.idx];
+          own[q.idx] = sp ^ msc_acc<S>(sp, oth[q.i1], oth[q.idx], oth[q.i3], oth[q.i4], c4[0], c8[0]);
+        }
+        __syncthreads();
+      }
+    }
   } else if (Nh <= 9 * kStreamsPerUnit) {
     // At most 9 sites per stream and half sweep (L <= 48), words dealt, not blocks: stream t's
     // n_t sites take the consecutive words (2s + h) n_t + k, k < n_t, of its stream, and word

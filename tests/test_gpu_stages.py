@@ -128,10 +128,15 @@ def test_ssc_sweep_stage_site_kernels(L, beta, uniform, nt):
                                                 (256, 64, 32, 0, 0, 0.44), (256, 32, 32, 0, 0, 0.0),
                                                 (64, 8, 0, 0, 0, 0.44), (16, 16, 0, 0, 0, 0.3),
                                                 (64, 16, 8, 256, -1, 0.6), (48, 8, 0, 0, 0, 1.0),
-                                                (96, 16, 0, 1024, 0, 0.44), (256, 16, 0, 0, 0, 0.44)])
+                                                (96, 16, 0, 1024, 0, 0.44), (256, 16, 0, 0, 0, 0.44),
+                                                # one site per stream and half sweep (L <= 16):
+                                                # a Philox block for two sweeps, odd sweep counts
+                                                (16, 32, 8, 256, -1, 0.44), (16, 64, 16, 512, -1, 0.7),
+                                                (10, 32, 0, 0, 0, 0.44), (16, 32, 32, 256, 1, 0.44),
+                                                (12, 64, 32, 128, -1, 1.0)])
 def test_msc_sweep_stage(L, w, sw, nt, bal, beta):
     rng = np.random.default_rng(L * w + sw)
-    R, IS, sweeps = 5 * w - 7, 123456789, 2
+    R, IS, sweeps = 5 * w - 7, 123456789, 3 if L <= 16 else 2
     rg = math.ceil(R / w)
     p = params(L, pa.MSC, w, eqsweeps=sweeps, slice_bits=sw, threads=nt, balance=bal)
     pop = msc_pop(L, w, R, rng)
