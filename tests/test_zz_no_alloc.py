@@ -56,6 +56,10 @@ PAIRS = [
     ((64, pa.MSC, 32, 3000, 6, 0), (24, pa.MSC, 32, 700, 4, 0)),
     ((96, pa.MSC, 64, 900, 4, 0), (32, pa.MSC, 64, 200, 7, 16)),
     ((256, pa.MSC, 64, 300, 2, 32), (128, pa.MSC, 32, 100, 3, 0)),  # one sublattice in LDS
+    # packed-byte SSC: the one-part walk (L=48), then the two-sweep step (L=24)
+    ((48, pa.SSC, 32, 300, 3, 2), (24, pa.SSC, 32, 100, 5, 2)),
+    # MSC with one site per stream and half sweep (L=16), odd sweep count
+    ((32, pa.MSC, 32, 900, 3, 0), (16, pa.MSC, 32, 300, 5, 0)),
 ]
 
 
