@@ -1490,6 +1490,10 @@ template <bool B>
 struct BoolC {
   static constexpr bool value = B;
 };
+template <int V>
+struct IntC {
+  static constexpr int value = V;
+};
 
 template <int NTMAX, int MB>
 __global__ __launch_bounds__(NTMAX) void k_sweep_ssc_q(const RunDesc* __restrict__ runs,
@@ -1980,6 +1984,10 @@ void k_sweep_ssc_p2(const RunDesc* __restrict__ runs, const int32_t* __restrict_
 // stream.  init: the freshly initialised population (buffer 0, energies only, no store).
 // FAST: the caller guarantees N/2 % 512 == 0 and nb <= NT / 128 (the register-cached path),
 // so the other paths are not compiled in and their registers do not count.
+// Ld2 / 2^a for the largest power of two 2^a dividing Ld2 (its odd part): the period in k of
+// the column of site t + 128 k when Ld2 < 128.
+__device__ __forceinline__ int msc_period(int Ld2) { return Ld2 >> __builtin_ctz((unsigned)Ld2); }
+
 template <typename W, typename S, int NT, bool FAST = false>
 __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const RunState* st, int g, int qs, uint32_t s_begin,
                          uint32_t s_end, bool init, bool energy, S* lat, int* accE, int* accM) {
@@ -2100,63 +2108,82 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
           __syncthreads();
         }
       }
-    } else if (Ld2 < 128 && 384 % Ld2 == 0 && ((384 / Ld2) & 1) == 0) {
-      // 3 x 128 an even multiple of Ld2 below 128 (L = 96, 192): site t + 128 k of stream t lies in
-      // column (t + 128 k) mod Ld2, which repeats every third k while the row advances by an
-      // even number, so the horizontal neighbour's offset depends only on (k mod 3, half);
-      // k = 4 (part + P j) + jj gives k mod 3 = (part + P j + jj) mod 3.  Vertical
-      // neighbours wrap only above the stream's first site and below its last, which holds
-      // only while Ld2 < 128 (at L = 384 the first two sites of a stream share row 0).
-      constexpr int B = (int)sizeof(S);
-      int ec[2][3];
+    } else if (Ld2 < 128 && (msc_period(Ld2) == 3 ||
+                             (sizeof(W) == 4 && (msc_period(Ld2) == 5 || msc_period(Ld2) == 7)))) {
+      // Ld2 = PER 2^a below 128 with PER = 3, 5, 7 (L = 96, 192; 160; 224; 5 and 7 for 32-bit
+      // words only, whose unrolled PER-block steps the other widths do not carry): site t + 128 k of
+      // stream t lies in column (t + 128 k) mod Ld2, which repeats every PER-th k while the row
+      // advances by 128 PER / Ld2 = 2^(7-a), an even number, so the horizontal neighbour's
+      // offset depends only on (k mod PER, half); k = 4 (part + P j) + jj with j a multiple of
+      // PER plus jr gives k mod PER = (4 part + 4 P jr + jj) mod PER.  Vertical neighbours wrap
+      // only above the stream's first site and below its last, which holds only while
+      // Ld2 < 128 (at L = 384 the first two sites of a stream share row 0).
+      auto period = [&](auto per_c) __attribute__((always_inline)) {
+        constexpr int PER = decltype(per_c)::value;
+        constexpr int B = (int)sizeof(S);
+        int ec[2][PER];
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const Site q = site_at(t + kStreamsPerUnit * c, h, d);
-          ec[h][c] = (q.i1 - q.idx) * B;
-        }
-      const int rot = part % 3;
-      int er[2][3];
+          for (int c = 0; c < PER; ++c) {
+            const Site q = site_at(t + kStreamsPerUnit * c, h, d);
+            ec[h][c] = (q.i1 - q.idx) * B;
+          }
+        const int rot = (4 * part) % PER;
+        int er[2][PER];
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int y = 0; y < 3; ++y)
-          er[h][y] = rot == 0 ? ec[h][y] : rot == 1 ? ec[h][(y + 1) % 3] : ec[h][(y + 2) % 3];
-      const int dV = Ld2 * B, dN = Nh * B;
-      const int top = t < Ld2 ? dN : 0, bot = t >= kStreamsPerUnit - Ld2 ? dN : 0;
-      for (uint32_t s = s_begin; s < s_end; ++s) {
+          for (int y = 0; y < PER; ++y) {
+            int v = ec[h][y];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          uint8_t* own = (uint8_t*)(lat + h * Nh);
-          const uint8_t* oth = (const uint8_t*)(lat + (h ^ 1) * Nh);
-          for (int j0 = 0; part + P * j0 < nb; j0 += 3) {
+            for (int r = 1; r < PER; ++r)
+              if (rot == r) v = ec[h][(y + r) % PER];
+            er[h][y] = v;
+          }
+        const int dV = Ld2 * B, dN = Nh * B;
+        const int top = t < Ld2 ? dN : 0, bot = t >= kStreamsPerUnit - Ld2 ? dN : 0;
+        for (uint32_t s = s_begin; s < s_end; ++s) {
 #pragma unroll
-            for (int jr = 0; jr < 3; ++jr) {
-              const int m = part + P * (j0 + jr);
-              if (m < nb) {
-                const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
-                uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
-                S c4[4], c8[4];
-                lcg_masks<S, 4>(ran, T4, T8, c4, c8);
-                const int o0 = (t + 512 * m) * B;
-                const int up = m == 0 ? top : 0, dn = m == nb - 1 ? bot : 0;
+          for (int h = 0; h < 2; ++h) {
+            uint8_t* own = (uint8_t*)(lat + h * Nh);
+            const uint8_t* oth = (const uint8_t*)(lat + (h ^ 1) * Nh);
+            for (int j0 = 0; part + P * j0 < nb; j0 += PER) {
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj) {
-                  const int o = o0 + jj * 128 * B;
-                  const int o1 = o + er[h][(P * jr + jj) % 3];
-                  const int o3 = o - dV + (jj == 0 ? up : 0);
-                  const int o4 = o + dV - (jj == 3 ? dn : 0);
-                  const S sp = *(const S*)(own + o);
-                  *(S*)(own + o) = sp ^ msc_acc<S>(sp, *(const S*)(oth + o1), *(const S*)(oth + o),
-                                                   *(const S*)(oth + o3), *(const S*)(oth + o4),
-                                                   c4[jj], c8[jj]);
+              for (int jr = 0; jr < PER; ++jr) {
+                const int m = part + P * (j0 + jr);
+                if (m < nb) {
+                  const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h) * nb + m);
+                  uint32_t ran[4] = {JA * rb.x + JC, JA * rb.y + JC, JA * rb.z + JC, JA * rb.w + JC};
+                  S c4[4], c8[4];
+                  lcg_masks<S, 4>(ran, T4, T8, c4, c8);
+                  const int o0 = (t + 512 * m) * B;
+                  const int up = m == 0 ? top : 0, dn = m == nb - 1 ? bot : 0;
+#pragma unroll
+                  for (int jj = 0; jj < 4; ++jj) {
+                    const int o = o0 + jj * 128 * B;
+                    const int o1 = o + er[h][(4 * P * jr + jj) % PER];
+                    const int o3 = o - dV + (jj == 0 ? up : 0);
+                    const int o4 = o + dV - (jj == 3 ? dn : 0);
+                    const S sp = *(const S*)(own + o);
+                    *(S*)(own + o) = sp ^ msc_acc<S>(sp, *(const S*)(oth + o1), *(const S*)(oth + o),
+                                                     *(const S*)(oth + o3), *(const S*)(oth + o4),
+                                                     c4[jj], c8[jj]);
+                  }
                 }
               }
             }
+            __syncthreads();
           }
-          __syncthreads();
         }
+      };
+      const int per = msc_period(Ld2);
+      if constexpr (sizeof(W) == 4) {
+        if (per == 3) period(IntC<3>{});
+        else if (per == 5) period(IntC<5>{});
+        else period(IntC<7>{});
+      } else {
+        period(IntC<3>{});
       }
     } else {
       for (uint32_t s = s_begin; s < s_end; ++s) {

@@ -133,7 +133,12 @@ def test_ssc_sweep_stage_site_kernels(L, beta, uniform, nt):
                                                 # a Philox block for two sweeps, odd sweep counts
                                                 (16, 32, 8, 256, -1, 0.44), (16, 64, 16, 512, -1, 0.7),
                                                 (10, 32, 0, 0, 0, 0.44), (16, 32, 32, 256, 1, 0.44),
-                                                (12, 64, 32, 128, -1, 1.0)])
+                                                (12, 64, 32, 128, -1, 1.0),
+                                                # columns repeating every 3rd, 5th, 7th site of
+                                                # a stream (L = 192, 160, 224)
+                                                (192, 32, 16, 0, 0, 0.44), (160, 32, 0, 0, 0, 0.44),
+                                                (160, 64, 32, 256, 1, 0.3), (224, 32, 16, 512, -1, 0.7),
+                                                (224, 16, 8, 1024, -1, 0.44)])
 def test_msc_sweep_stage(L, w, sw, nt, bal, beta):
     rng = np.random.default_rng(L * w + sw)
     R, IS, sweeps = 5 * w - 7, 123456789, 3 if L <= 16 else 2
