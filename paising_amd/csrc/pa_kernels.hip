@@ -1988,7 +1988,7 @@ void k_sweep_ssc_p2(const RunDesc* __restrict__ runs, const int32_t* __restrict_
 // the column of site t + 128 k when Ld2 < 128.
 __device__ __forceinline__ int msc_period(int Ld2) { return Ld2 >> __builtin_ctz((unsigned)Ld2); }
 
-template <typename W, typename S, int NT, bool FAST = false>
+template <typename W, typename S, int NT, bool FAST = false, bool PER57 = true>
 __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const RunState* st, int g, int qs, uint32_t s_begin,
                          uint32_t s_end, bool init, bool energy, S* lat, int* accE, int* accM) {
   constexpr int w = (int)sizeof(W) * 8;
@@ -2109,9 +2109,12 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
         }
       }
     } else if (Ld2 < 128 && (msc_period(Ld2) == 3 ||
-                             (sizeof(W) == 4 && (msc_period(Ld2) == 5 || msc_period(Ld2) == 7)))) {
+                             (PER57 && sizeof(W) == 4 && (msc_period(Ld2) == 5 || msc_period(Ld2) == 7)))) {
       // Ld2 = PER 2^a below 128 with PER = 3, 5, 7 (L = 96, 192; 160; 224; 5 and 7 for 32-bit
-      // words only, whose unrolled PER-block steps the other widths do not carry): site t + 128 k of
+      // words in the plain sweep kernel only: the unrolled PER-block steps are long, and the
+      // balanced kernel, whose registers the headline shape uses to the limit, does not carry
+      // them; these lattices' 32-bit slices exceed 80 KiB and take 1024-thread plain
+      // workgroups): site t + 128 k of
       // stream t lies in column (t + 128 k) mod Ld2, which repeats every PER-th k while the row
       // advances by 128 PER / Ld2 = 2^(7-a), an even number, so the horizontal neighbour's
       // offset depends only on (k mod PER, half); k = 4 (part + P j) + jj with j a multiple of
@@ -2178,7 +2181,7 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
         }
       };
       const int per = msc_period(Ld2);
-      if constexpr (sizeof(W) == 4) {
+      if constexpr (PER57 && sizeof(W) == 4) {
         if (per == 3) period(IntC<3>{});
         else if (per == 5) period(IntC<5>{});
         else period(IntC<7>{});
@@ -2576,7 +2579,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FAST ? 6 : 4
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
-      msc_unit<W, S, NT, FAST>(d, st, g, qs, (uint32_t)s0, (uint32_t)s1, false, !publish, lat, accE, accM);
+      msc_unit<W, S, NT, FAST, false>(d, st, g, qs, (uint32_t)s0, (uint32_t)s1, false, !publish, lat, accE, accM);
       if (publish) {
         __syncthreads();
         if (threadIdx.x == 0) __hip_atomic_store(flag, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
