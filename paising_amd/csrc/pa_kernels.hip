@@ -1988,7 +1988,7 @@ void k_sweep_ssc_p2(const RunDesc* __restrict__ runs, const int32_t* __restrict_
 // the column of site t + 128 k when Ld2 < 128.
 __device__ __forceinline__ int msc_period(int Ld2) { return Ld2 >> __builtin_ctz((unsigned)Ld2); }
 
-template <typename W, typename S, int NT, bool FAST = false, bool PER57 = true>
+template <typename W, typename S, int NT, bool FAST = false, bool PLAIN = true>
 __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const RunState* st, int g, int qs, uint32_t s_begin,
                          uint32_t s_end, bool init, bool energy, S* lat, int* accE, int* accM) {
   constexpr int w = (int)sizeof(W) * 8;
@@ -2039,6 +2039,38 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
       const int o0 = (t + 512 * part) * B, o3b = o0 + 3 * 128 * B;
       const int up0 = o0 < dV ? o0 - dV + dN : o0 - dV;
       const int dn3 = o3b + dV >= dN ? o3b + dV - dN : o3b + dV;
+      if constexpr (PLAIN && !FAST) {
+        if (nb == 1 && P == 4) {
+          // One Philox block per stream and half sweep (L = 32) over four parts: part jj takes
+          // the block's word jj (each part evaluates the block), four times the working waves
+          // of one part per stream (the rate of these small lattices follows their waves).
+          const int jj = part;
+          const int o = (t + 128 * jj) * B;
+          const int o3 = jj == 0 ? (t * B < dV ? t * B - dV + dN : t * B - dV) : o - dV;
+          const int o4 = jj == 3 ? (o + dV >= dN ? o + dV - dN : o + dV) : o + dV;
+          int dh[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) dh[h] = ((ty0 + jj * d.q128 + h) & 1) ? dR : dL;
+          for (uint32_t s = s_begin; s < s_end; ++s) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              uint8_t* own = (uint8_t*)(lat + h * Nh);
+              const uint8_t* oth = (const uint8_t*)(lat + (h ^ 1) * Nh);
+              const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h));
+              uint32_t ran[1] = {JA * u4_word(rb, jj) + JC};
+              S c4[1], c8[1];
+              if constexpr (SW >= 16) lcg_masks_split<S>(ran[0], T4, T8, c4[0], c8[0]);
+              else lcg_masks<S, 1>(ran, T4, T8, c4, c8);
+              const S sp = *(const S*)(own + o);
+              *(S*)(own + o) = sp ^ msc_acc<S>(sp, *(const S*)(oth + o + dh[h]),
+                                               *(const S*)(oth + o), *(const S*)(oth + o3),
+                                               *(const S*)(oth + o4), c4[0], c8[0]);
+              __syncthreads();
+            }
+          }
+          goto unit_done;
+        }
+      }
       int d1[2][4];
 #pragma unroll
       for (int h = 0; h < 2; ++h)
@@ -2109,7 +2141,7 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
         }
       }
     } else if (Ld2 < 128 && (msc_period(Ld2) == 3 ||
-                             (PER57 && sizeof(W) == 4 && (msc_period(Ld2) == 5 || msc_period(Ld2) == 7)))) {
+                             (PLAIN && sizeof(W) == 4 && (msc_period(Ld2) == 5 || msc_period(Ld2) == 7)))) {
       // Ld2 = PER 2^a below 128 with PER = 3, 5, 7 (L = 96, 192; 160; 224; 5 and 7 for 32-bit
       // words in the plain sweep kernel only: the unrolled PER-block steps are long, and the
       // balanced kernel, whose registers the headline shape uses to the limit, does not carry
@@ -2181,7 +2213,7 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
         }
       };
       const int per = msc_period(Ld2);
-      if constexpr (PER57 && sizeof(W) == 4) {
+      if constexpr (PLAIN && sizeof(W) == 4) {
         if (per == 3) period(IntC<3>{});
         else if (per == 5) period(IntC<5>{});
         else period(IntC<7>{});
@@ -2306,6 +2338,7 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
       }
     }
   }
+unit_done:
   if (energy) msc_energy_slice<S>(lat, L, Ld2, Nh, accE, accM, qs * SW, d.IE, d.M, (size_t)g * w);
   if (!init) {
     __syncthreads();

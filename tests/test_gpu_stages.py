@@ -138,7 +138,10 @@ def test_ssc_sweep_stage_site_kernels(L, beta, uniform, nt):
                                                 # a stream (L = 192, 160, 224)
                                                 (192, 32, 16, 0, 0, 0.44), (160, 32, 0, 0, 0, 0.44),
                                                 (160, 64, 32, 256, 1, 0.3), (224, 32, 16, 512, -1, 0.7),
-                                                (224, 16, 8, 1024, -1, 0.44)])
+                                                (224, 16, 8, 1024, -1, 0.44),
+                                                # L = 32 over four parts (one block word each)
+                                                (32, 32, 0, 512, -1, 0.44), (32, 64, 32, 512, -1, 0.6),
+                                                (32, 32, 16, 512, 0, 0.3), (32, 16, 8, 512, -1, 1.0)])
 def test_msc_sweep_stage(L, w, sw, nt, bal, beta):
     rng = np.random.default_rng(L * w + sw)
     R, IS, sweeps = 5 * w - 7, 123456789, 3 if L <= 16 else 2
