@@ -409,6 +409,13 @@ struct RunHost {
   size_t sweep_lds() const {
     return (size_t)rpw * N + (ssc_p == 1 ? (size_t)32 * 16 * ceil_div(N / 2, 128) : 0);
   }
+  // LDS of an MSC sweep unit: its lattice slice (one sublattice for the half-resident form),
+  // plus for L = 32 over 512 threads (four parts, one word of the stream's block each) the
+  // Philox words handed between the parts: 2 x 4 half sweeps x 4 words x 128 streams
+  size_t msc_unit_lds() const {
+    const size_t slice = (size_t)(half ? N / 2 : N) * (sw / 8);
+    return slice + (!half && N / 2 == 512 && nt == 512 ? (size_t)2 * 4 * 4 * 128 * 4 : 0);
+  }
   size_t resample_lds() const {
     // parents' chunks plus one zero chunk
     if (rs_win) return kWinLds;
@@ -1068,7 +1075,7 @@ struct Batch {
           acc += h.cap_units;
           acc8 += h.blocks(kKSweep);
           const size_t l = h.p.coding == PA_SSC ? (size_t)h.N
-                           : (size_t)(h.half ? h.N / 2 : h.N) * (h.sw / 8);
+                           : h.msc_unit_lds();
           lds = std::max(lds, l);
           lds8 = std::max(lds8, h.path == 8 || h.path == 9 ? h.sweep_lds() : 0);
         }
@@ -2221,7 +2228,7 @@ extern "C" int pa_stage_sweep_noalloc(const pa_params* p, int R, uint64_t is, do
   hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(64), 0, s, t, dev);
   HIPCHECK(hipGetLastError());
   const size_t lds = h.p.coding == PA_SSC ? (size_t)h.N
-                     : (size_t)(h.half ? h.N / 2 : h.N) * (h.sw / 8);
+                     : h.msc_unit_lds();
   if (h.p.coding == PA_MSC && h.half)
     HIPCHECK(pa_launch_step(16, PA_MSC, h.w, 32, 1024, units * (h.w / 32), lds, s, &dev->desc,
                             dev->offs, 1));

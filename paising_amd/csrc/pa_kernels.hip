@@ -2042,8 +2042,11 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
       if constexpr (PLAIN && !FAST) {
         if (nb == 1 && P == 4) {
           // One Philox block per stream and half sweep (L = 32) over four parts: part jj takes
-          // the block's word jj (each part evaluates the block), four times the working waves
-          // of one part per stream (the rate of these small lattices follows their waves).
+          // the block's word jj, four times the working waves of one part per stream (the rate
+          // of these small lattices follows their waves).  The four blocks of two sweeps are
+          // evaluated once, one per part, two sweeps ahead during the first half sweep, and
+          // handed over in LDS after the lattice slice: two buffers of 4 half sweeps x 4 words
+          // x 128 streams (the host adds their 16 KiB to this shape's LDS).
           const int jj = part;
           const int o = (t + 128 * jj) * B;
           const int o3 = jj == 0 ? (t * B < dV ? t * B - dV + dN : t * B - dV) : o - dV;
@@ -2051,13 +2054,35 @@ __device__ __attribute__((always_inline)) void msc_unit(const RunDesc& d, const 
           int dh[2];
 #pragma unroll
           for (int h = 0; h < 2; ++h) dh[h] = ((ty0 + jj * d.q128 + h) & 1) ? dR : dL;
-          for (uint32_t s = s_begin; s < s_end; ++s) {
+          uint32_t* wb = (uint32_t*)(lat + N);
+          // part p: the block of half sweep 2 g0 + p (sweep g0 + p / 2) into buffer bf
+          auto fill = [&](uint32_t g0, int bf) __attribute__((always_inline)) {
+            const uint32_t sp = g0 + (uint32_t)(part >> 1);
+            if (sp >= s_begin && sp < s_end) {
+              const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * g0 + part));
+              uint32_t* q = wb + (size_t)((bf * 4 + part) * 4) * kStreamsPerUnit + t;
+              q[0] = rb.x;
+              q[kStreamsPerUnit] = rb.y;
+              q[2 * kStreamsPerUnit] = rb.z;
+              q[3 * kStreamsPerUnit] = rb.w;
+            }
+          };
+          const uint32_t first = s_begin & ~1u;
+          fill(first, 0);
+          __syncthreads();
+          int bf = 0;
+          for (uint32_t s0 = first; s0 < s_end; s0 += 2, bf ^= 1) {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int hs = 0; hs < 4; ++hs) {
+              // the next two sweeps' blocks go to the other buffer, last read two sweeps ago
+              if (hs == 0 && s0 + 2 < s_end) fill(s0 + 2, bf ^ 1);
+              const uint32_t s = s0 + (hs >> 1);
+              if (s < s_begin || s >= s_end) continue;  // uniform
+              const int h = hs & 1;
               uint8_t* own = (uint8_t*)(lat + h * Nh);
               const uint8_t* oth = (const uint8_t*)(lat + (h ^ 1) * Nh);
-              const U4 rb = stream_block(seed, sub0 + t, (uint64_t)(2 * s + h));
-              uint32_t ran[1] = {JA * u4_word(rb, jj) + JC};
+              const uint32_t word = wb[(size_t)((bf * 4 + hs) * 4 + jj) * kStreamsPerUnit + t];
+              uint32_t ran[1] = {JA * word + JC};
               S c4[1], c8[1];
               if constexpr (SW >= 16) lcg_masks_split<S>(ran[0], T4, T8, c4[0], c8[0]);
               else lcg_masks<S, 1>(ran, T4, T8, c4, c8);

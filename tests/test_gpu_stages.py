@@ -144,7 +144,7 @@ def test_ssc_sweep_stage_site_kernels(L, beta, uniform, nt):
                                                 (32, 32, 16, 512, 0, 0.3), (32, 16, 8, 512, -1, 1.0)])
 def test_msc_sweep_stage(L, w, sw, nt, bal, beta):
     rng = np.random.default_rng(L * w + sw)
-    R, IS, sweeps = 5 * w - 7, 123456789, 3 if L <= 16 else 2
+    R, IS, sweeps = 5 * w - 7, 123456789, 3 if L <= 32 else 2
     rg = math.ceil(R / w)
     p = params(L, pa.MSC, w, eqsweeps=sweeps, slice_bits=sw, threads=nt, balance=bal)
     pop = msc_pop(L, w, R, rng)
