@@ -659,8 +659,8 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     // 48 3390 vs 3027, 2113 vs 1471; 56 3358 vs 3065)
     // L = 32 (one Philox block per stream and half sweep), lone runs: 512 threads, the four
     // parts of a stream taking one word of its block each (R = 5000 / 20000 / 80000,
-    // attempts/ns, against 256 threads with one working part: 1810 vs 1273, 3763 vs 3352,
-    // 4954 vs 4681).  In a batch the other runs bring the waves, and a variant of its own
+    // attempts/ns, against 256 threads with one working part: 2059 vs 1270, 4091 vs 3326,
+    // 5427 vs 4710).  In a batch the other runs bring the waves, and a variant of its own
     // would add a launch per step (the small-run mix: 662 against 693-723 runs/s).
     if (!p.threads && !half && Nh == 512 && co_work == 0) h.nt = 512;
     if (!p.threads && h.w >= 32 && Nh > 4 * 128 && Nh <= 13 * 128) {
