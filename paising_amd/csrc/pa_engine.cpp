@@ -584,9 +584,12 @@ int plan_run(const pa_params& p_in, RunHost& h, double cap_sigma, int64_t co_wor
     if (h.sweep_lds() + 1024 > kLdsLimit)
       return fail(PA_ERR_INVALID, "packed-byte SSC workgroup exceeds LDS");
     h.nt = nt8;
-    // one part and 257..384 sites per sublattice (L = 24): the two-sweep step
-    // (k_sweep_ssc_p2, at most 512 threads)
-    h.path = h.ssc_p == 1 && nt8 <= 512 && Nh > 256 && Nh <= 384 ? 9 : 8;
+    // one part and 257..384 sites per sublattice (L = 24): the two-sweep step with the lanes
+    // sorted by dword count (k_sweep_ssc_p2, at most 512 threads, whole wavefronts per class)
+    h.path = h.ssc_p == 1 && nt8 <= 512 && Nh > 256 && Nh <= 384 &&
+                     (h.rpw * ((Nh - 256) / 4)) % 64 == 0
+                 ? 9
+                 : 8;
   } else if (h.nt != 128 && h.nt != 256 && h.nt != 512 && !(h.nt == 1024 && p.coding == PA_MSC)) {
     // 1024-thread workgroups (MSC only): twice the waves per CU for lattices whose slice
     // leaves room for one workgroup per CU

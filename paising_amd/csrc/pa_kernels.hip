@@ -1773,81 +1773,69 @@ __device__ __forceinline__ void ssc_p_sweeps_nq1(uint8_t* lat8, const RunDesc& d
   }
 }
 
-// One part, 257 to 512 sites per sublattice (L = 24, 32): quad q has n_q = C or C - 1
-// dwords per half sweep (C = ceil(N/2 / 128) = 3 or 4), so two sweeps (half sweeps
+// One part, 257 to 384 sites per sublattice (L = 24): quad q has n_q = C = 3 dwords per half
+// sweep for q < qb = (N/2 - 256) / 4 and 2 for the others, so two sweeps (half sweeps
 // j = 4J .. 4J + 3) take words 4 J n_q .. 4 J n_q + 4 n_q - 1 of its streams, exactly the
-// Philox blocks J n_q + p, p < n_q; which word of which block feeds dword k of half sweep hs is
-// known at compile time for both lane classes (word hs n_q + k of the two sweeps).  Blocks
-// are evaluated in order, each before the first half sweep whose C-class dwords need it; a
-// (C-1)-class lane lags at most one block behind (its block C - 1 is evaluated and unused).
-// Where the classes read different words (MIXED: C = 3 at L = 24) a dword's four words are
-// selected per lane.  Geometry from the per-workgroup table; a dword a lane does not own
-// reads dword 0 and stores nothing.  Same update and wavefront-scope fences as the one-part
-// walk of ssc_p_sweeps, none of its loop control.
-template <bool WIDE, int C, bool MIXED>
-__device__ __forceinline__ void ssc_p_sweeps_c(uint8_t* lat8, const uint4* geo, const RunDesc& d,
-                                               uint32_t nsw, uint64_t seed, uint64_t sub, int q,
-                                               int nq, bool walk, const uint8_t* tb,
-                                               const uint64_t* tabw) {
+// Philox blocks J n_q + p, p < n_q, and dword k of half sweep hs takes word hs n_q + k of the
+// two sweeps: known at compile time per class.  The workgroup's RPW replicas give their
+// three-dword quads to the first RPW qb lanes and their two-dword quads to the rest, RPW qb a
+// multiple of 64, so every wavefront holds one class: three Philox quads and twelve dword
+// updates per two sweeps in the first, two and eight in the others (lanes of both classes in
+// one wavefront would each take the larger schedule and select words per class).  A replica
+// spans wavefronts, so the half sweeps end in workgroup barriers (outside the class
+// branches).  Geometry from the per-workgroup table; update as in k_sweep_ssc_q.
+template <bool WIDE>
+__device__ __forceinline__ void ssc_p_sweeps_sorted(uint8_t* lat8, const uint4* geo,
+                                                    const RunDesc& d, uint32_t nsw, uint64_t seed,
+                                                    uint64_t sub, int q, bool big, bool walk,
+                                                    const uint8_t* tb, const uint64_t* tabw) {
   const int Nh = d.Nh;
-  const bool big = nq == C;
-  const uint64_t n64 = (uint64_t)(uint32_t)nq;
-  for (uint32_t s = 0; s < nsw; s += 2) {
-    U4 blk[C][4];
+  U4 blk[3][4];  // the class's blocks of the current two sweeps (two used by the 2-dword class)
+  auto step = [&](auto c_c, uint32_t s0, int hs) __attribute__((always_inline)) {
+    constexpr int C = decltype(c_c)::value;
+    const int h = hs & 1;
+    uint8_t* own8 = lat8 + h * Nh;
+    const uint8_t* oth8 = lat8 + (h ^ 1) * Nh;
+#pragma unroll
+    for (int p = 0; p < C; ++p)
+      if ((4 * p) / C == hs) {
+        U4 rq[4];
+        stream_block_quad(seed, sub, (uint64_t)(s0 >> 1) * C + p, rq);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) blk[p][j] = rq[j];
+      }
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int wb = hs * C + k;
+      const uint4 e = geo[k * 32 + q];
+      const int o = (int)e.w;
+      const uint32_t S = *(const uint32_t*)(own8 + o);
+      const uint32_t D0 = *(const uint32_t*)(oth8 + o);
+      const uint32_t Dx = *(const uint32_t*)(oth8 + (h ? (e.x >> 16) : (e.x & 0xFFFFu)));
+      const uint32_t Du = *(const uint32_t*)(oth8 + (e.y & 0xFFFFu));
+      const uint32_t Dd = *(const uint32_t*)(oth8 + (e.y >> 16));
+      const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, h ? kSelSum - e.z : e.z);
+      *(uint32_t*)(own8 + o) = ssc_dword_update<WIDE>(
+          S, D0, Dh, Du, Dd, u4_word(blk[wb >> 2][0], wb & 3), u4_word(blk[wb >> 2][1], wb & 3),
+          u4_word(blk[wb >> 2][2], wb & 3), u4_word(blk[wb >> 2][3], wb & 3), tb, tabw);
+    }
+  };
+  for (uint32_t s0 = 0; s0 < nsw; s0 += 2) {
 #pragma unroll
     for (int hs = 0; hs < 4; ++hs) {
-      if (s + (hs >> 1) >= nsw) continue;  // an odd sweep count ends mid-step (uniform)
-      const int h = hs & 1;
-      uint8_t* own8 = lat8 + h * Nh;
-      const uint8_t* oth8 = lat8 + (h ^ 1) * Nh;
+      if (s0 + (hs >> 1) >= nsw) continue;  // an odd sweep count ends mid-step (uniform)
       if (walk) {
-#pragma unroll
-        for (int p = 0; p < C; ++p)
-          if ((4 * p) / C == hs) {
-            U4 rq[4];
-            stream_block_quad(seed, sub, (uint64_t)(s >> 1) * n64 + p, rq);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) blk[p][j] = rq[j];
-          }
-#pragma unroll
-        for (int k = 0; k < C; ++k) {
-          const int wb = hs * C + k, ws = hs * (C - 1) + k;
-          uint32_t v[4];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const uint32_t vb = u4_word(blk[wb >> 2][j], wb & 3);
-            if constexpr (MIXED) {
-              if (k < C - 1 && (ws >> 2 != wb >> 2 || (ws & 3) != (wb & 3)))
-                v[j] = big ? vb : u4_word(blk[ws >> 2][j], ws & 3);
-              else
-                v[j] = vb;
-            } else {
-              v[j] = vb;
-            }
-          }
-          // geometry from the table (a dword the lane does not own has dword 0's entry)
-          const uint4 e = geo[k * 32 + q];
-          const int o = (int)e.w;
-          const uint32_t S = *(const uint32_t*)(own8 + o);
-          const uint32_t D0 = *(const uint32_t*)(oth8 + o);
-          const uint32_t Dx = *(const uint32_t*)(oth8 + (h ? (e.x >> 16) : (e.x & 0xFFFFu)));
-          const uint32_t Du = *(const uint32_t*)(oth8 + (e.y & 0xFFFFu));
-          const uint32_t Dd = *(const uint32_t*)(oth8 + (e.y >> 16));
-          const uint32_t Dh = __builtin_amdgcn_perm(Dx, D0, h ? kSelSum - e.z : e.z);
-          const uint32_t S2 =
-              ssc_dword_update<WIDE>(S, D0, Dh, Du, Dd, v[0], v[1], v[2], v[3], tb, tabw);
-          if (k < C - 1 || !MIXED || big) *(uint32_t*)(own8 + o) = S2;
-        }
+        if (big) step(IntC<3>{}, s0, hs);
+        else step(IntC<2>{}, s0, hs);
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __syncthreads();
     }
   }
 }
 
-// MODE 0: the stream walk of any L % 8 == 0 (k_sweep_ssc_p); MODE 1: two sweeps per step for
-// one part and 257..384 sites per sublattice (k_sweep_ssc_p2, its own register budget).
+// MODE 0: the stream walk of any L % 8 == 0 (k_sweep_ssc_p); MODE 1: two sweeps per step with
+// class-sorted lanes for one part and 257..384 sites per sublattice (k_sweep_ssc_p2, its own
+// register budget).
 template <int MODE>
 __device__ __forceinline__ void ssc_p_body(const RunDesc* __restrict__ runs,
                                            const int32_t* __restrict__ offs, int nruns) {
@@ -1862,7 +1850,8 @@ __device__ __forceinline__ void ssc_p_body(const RunDesc* __restrict__ runs,
   const int B0 = (blockIdx.x - off[r]) * RPW;
   const int Nh0 = d.Nh;
   if (RPW < 1 || RPW > 32 ||  // a launch shape the plan never makes: refuse (host error bit 2)
-      (MODE == 1 && !(P == 1 && Nh0 > 256 && Nh0 <= 384))) {
+      (MODE == 1 && !(P == 1 && Nh0 > 256 && Nh0 <= 384 && Nh0 % 4 == 0 &&
+                      (RPW * ((Nh0 - 256) >> 2)) % 64 == 0))) {
     if (threadIdx.x == 0) atomicOr(&st->err, 2);
     return;
   }
@@ -1916,8 +1905,19 @@ __device__ __forceinline__ void ssc_p_body(const RunDesc* __restrict__ runs,
   __syncthreads();
   const uint8_t* tb = (const uint8_t*)tabn;
   if constexpr (MODE == 1) {
-    if (wide) ssc_p_sweeps_c<true, 3, true>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, nq, walk, tb, tabw);
-    else ssc_p_sweeps_c<false, 3, true>(lat8, geo, d, d.eqsweeps, d.seed, sub, q, nq, walk, tb, tabw);
+    // sorted lanes: thread i < RPW qb takes three-dword quad i % qb of replica i / qb, the
+    // others two-dword quad qb + j % (32 - qb) of replica j / (32 - qb), j = i - RPW qb
+    const int qb = (Nh - 256) >> 2, nbig = RPW * qb, tid = (int)threadIdx.x;
+    const bool big = tid < nbig;
+    const int j = tid - nbig;
+    const int slot2 = big ? tid / qb : j / (32 - qb);
+    const int q2 = big ? tid % qb : qb + j % (32 - qb);
+    const int B2 = B0 + slot2;
+    const bool walk2 = tid < RPW * 32 && B2 < Rn;
+    uint8_t* lat82 = (uint8_t*)latq + (size_t)(walk2 ? slot2 : 0) * N;
+    const uint64_t sub2 = st->IS + (uint64_t)st->R + (uint64_t)B2 * kStreamsPerUnit + 4 * q2;
+    if (wide) ssc_p_sweeps_sorted<true>(lat82, geo, d, d.eqsweeps, d.seed, sub2, q2, big, walk2, tb, tabw);
+    else ssc_p_sweeps_sorted<false>(lat82, geo, d, d.eqsweeps, d.seed, sub2, q2, big, walk2, tb, tabw);
   } else if (tab && Nh <= 128) {
     if (wide) ssc_p_sweeps_nq1<true>(lat8, d, d.eqsweeps, d.seed, sub, q, walk, tb, tabw);
     else ssc_p_sweeps_nq1<false>(lat8, d, d.eqsweeps, d.seed, sub, q, walk, tb, tabw);
@@ -1969,9 +1969,9 @@ void k_sweep_ssc_p(const RunDesc* __restrict__ runs, const int32_t* __restrict__
   ssc_p_body<0>(runs, offs, nruns);
 }
 
-// Up to 512 threads and 168 VGPRs (three waves per SIMD): the two-sweep step holds two
-// Philox quads and the geometry of up to four dwords, which spill at 80 (six waves, the
-// budget of k_sweep_ssc_p) and at 128.
+// Up to 512 threads and 128 VGPRs (four waves per SIMD): the two-sweep step holds the Philox
+// quads of its class, which spill inside the sweep loop at 80 VGPRs (six waves, the budget of
+// k_sweep_ssc_p) and at 96 (five).
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 void k_sweep_ssc_p2(const RunDesc* __restrict__ runs, const int32_t* __restrict__ offs, int nruns) {
   ssc_p_body<1>(runs, offs, nruns);

@@ -58,7 +58,8 @@ def msc_pop(L, w, R, rng):
                                                # the two-sweep step (L = 24: three or two dwords
                                                # per quad; L = 32: four), both threshold forms
                                                (24, 0.44, 0, -256), (24, 6.0, 1, -64),
-                                               (24, 0.7, 1, -512), (32, 6.0, 0, -512)])
+                                               (24, 0.7, 1, -512), (32, 6.0, 0, -512),
+                                               (24, 6.0, 0, -256), (24, 1.0, 1, -128)])
 def test_ssc_sweep_stage(L, beta, uniform, nt):
     rng = np.random.default_rng(L + abs(nt))
     R, IS, sweeps = 37, 9876543210, 3
